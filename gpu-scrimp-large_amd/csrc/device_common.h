@@ -1,0 +1,86 @@
+// Device-side helpers shared by the statistics, seed and sweep kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+namespace scamp_amd {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kDiagsPerLane = 8;   // consecutive diagonals held by one lane
+constexpr int kBand = kWave * kDiagsPerLane;  // diagonals per wave band
+// Column-side arrays are padded so that the sweep's look-ahead loads and the
+// lanes of a partial last band never leave the allocation.
+constexpr int64_t kColumnPad = kBand + 8 * kDiagsPerLane;
+
+__host__ __device__ inline uint32_t f32_bits(float f) {
+  uint32_t b;
+  memcpy(&b, &f, sizeof(b));
+  return b;
+}
+__host__ __device__ inline float bits_f32(uint32_t b) {
+  float f;
+  memcpy(&f, &b, sizeof(f));
+  return f;
+}
+
+// Profile keys.  A 1NN / 1NN_INDEX entry is held on the device as one uint64
+// whose unsigned order is the order matches are ranked in: the FP32
+// correlation in an order-preserving encoding in the high word, the
+// bit-inverted index in the low word (so equal correlations keep the lower
+// index).  One 64-bit atomicMax merges a candidate whatever the order of
+// arrival.  The boundary converts to SCAMP's mp_entry layout
+// (src/common.h:70-74: correlation in the low word) on the way in and out.
+__host__ __device__ inline uint32_t enc_corr(float f) {
+  uint32_t b = f32_bits(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline float dec_corr(uint32_t e) {
+  uint32_t b = (e & 0x80000000u) ? (e & 0x7fffffffu) : ~e;
+  return bits_f32(b);
+}
+__host__ __device__ inline uint64_t make_key(float corr, uint32_t index) {
+  return (static_cast<uint64_t>(enc_corr(corr)) << 32) |
+         static_cast<uint32_t>(~index);
+}
+__host__ __device__ inline float key_corr(uint64_t key) {
+  return dec_corr(static_cast<uint32_t>(key >> 32));
+}
+__host__ __device__ inline uint32_t key_index(uint64_t key) {
+  return ~static_cast<uint32_t>(key);
+}
+
+// Filter thresholds.  A double correlation c can round to an FP32 value
+// >= the stored correlation f only if c >= f - 2^-25 (|f| <= 2); testing
+// c >= f - 2^-22 therefore never misses a candidate.  The FP32 form of the
+// sweep compares rounded values directly.
+template <typename T>
+struct Thresh;
+template <>
+struct Thresh<double> {
+  __device__ static double from_key_hi(uint32_t hi) {
+    return static_cast<double>(dec_corr(hi)) - 0x1p-22;
+  }
+};
+template <>
+struct Thresh<float> {
+  __device__ static float from_key_hi(uint32_t hi) { return dec_corr(hi); }
+};
+
+__device__ inline uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+}  // namespace scamp_amd

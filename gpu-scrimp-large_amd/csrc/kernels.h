@@ -1,0 +1,86 @@
+// Host-side launchers of the HIP kernels (implemented in prep_kernels.hip and
+// sweep_kernel.hip); used by the C ABI in scamp_capi.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scamp_amd {
+
+// ---- statistics (prep_kernels.hip) ----------------------------------------
+// T[n] raw series -> clean[n] (non-finite -> 0); means/inorms/df/dg[n-m+1].
+// Output arrays may be longer (padding): the pad is filled with mean 0,
+// inorm NaN, df 0, dg 0, clean 0.
+hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
+                        int64_t clean_len, double *means, double *inorms,
+                        double *df, double *dg, int64_t stat_len,
+                        hipStream_t s);
+
+// FP64 -> FP32 copies of the per-window update terms and inverse norms.
+hipError_t launch_to_f32(const double *src, float *dst, int64_t len,
+                         hipStream_t s);
+
+// ---- exact QT seeds (prep_kernels.hip) ------------------------------------
+// One task = `count` (<= 256) consecutive diagonals starting at diag0, all at
+// row `row`:  out[j] = sum_t (X[row+diag0+j+t]-muX[row+diag0+j]) *
+//                            (Y[row+t]-muY[row])
+// with compensated (Dot2) accumulation.
+struct SeedTask {
+  int64_t row;
+  int64_t diag0;
+  int64_t out_offset;
+  int32_t count;
+  int32_t pad;
+};
+hipError_t launch_seeds(const SeedTask *tasks, int64_t ntasks,
+                        const double *X, const double *muX, const double *Y,
+                        const double *muY, int64_t m, double *out,
+                        hipStream_t s);
+
+// ---- profile conversions (prep_kernels.hip) -------------------------------
+// SCAMP mp_entry (corr low word, index high word) <-> sortable keys.
+hipError_t launch_mp_to_keys(uint64_t *p, int64_t n, hipStream_t s);
+hipError_t launch_keys_to_mp(uint64_t *p, int64_t n, hipStream_t s);
+// float profile (1NN) <-> keys (index unused)
+hipError_t launch_f32_to_keys(const float *src, uint64_t *dst, int64_t n,
+                              hipStream_t s);
+hipError_t launch_keys_to_f32(const uint64_t *src, float *dst, int64_t n,
+                              hipStream_t s);
+// float matrix (MATRIX_SUMMARY) <-> order-preserving uint32
+hipError_t launch_f32_to_enc(float *p, int64_t n, hipStream_t s);
+hipError_t launch_enc_to_f32(float *p, int64_t n, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+
+// ---- the diagonal sweep (sweep_kernel.hip) --------------------------------
+enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3 };
+
+struct SweepArgs {
+  // column side X / row side Y update terms and inverse norms (FP64 or FP32,
+  // padded by kColumnPad entries)
+  const void *dfX, *dgX, *inX;
+  const void *dfY, *dgY, *inY;
+  // clean series and means (FP32 form: in-kernel FP64 re-anchoring)
+  const double *TX, *TY, *muX, *muY;
+  // FP64 form: exact seeds at row 0 (indexed by diagonal - d_lo) and at each
+  // band's first row (kBand entries per band)
+  const double *seed0, *seedB;
+  const int64_t *band_r0;
+  int64_t nX, nY;        // subsequence counts of X and Y
+  int64_t d_lo, d_hi;    // diagonal range of this pass
+  int64_t nbands;
+  int64_t m;
+  int64_t anchor;        // FP32 form: rows between re-anchors (multiple of 16)
+  uint64_t *keyX, *keyY; // 1NN / 1NN_INDEX profiles (columns, rows)
+  double *sumX, *sumY;   // SUM_THRESH profiles
+  uint32_t *mat;         // MATRIX_SUMMARY cells (order-preserving uint32)
+  int64_t mat_w, rpc, cpc;
+  double thr;
+  unsigned int *counter; // work counter, zeroed before launch
+};
+
+// cols/rows: which profiles the pass updates (for kMatrix: direct cells
+// (row, col) and transposed cells (col, row) respectively).
+hipError_t launch_sweep(const SweepArgs &a, bool fp32, SweepKind kind,
+                        bool cols, bool rows, int grid, hipStream_t s);
+
+}  // namespace scamp_amd

@@ -1,0 +1,259 @@
+// Statistics, exact QT seeds and profile-format conversions.
+//
+// Statistics replace compute_statistics_cpu (src/cpu_stats.cpp:123-160): the
+// reference runs sequential O(n) recurrences on one host thread before any
+// worker starts; here every window is summed directly (O(n*m) work, which the
+// GPU finishes in milliseconds) with compensated summation, so the result does
+// not accumulate error along the series.  The definitions are SCAMP's:
+//   window i has a non-finite sample  -> inorm NaN        (cpu_stats.cpp:11-29)
+//   SSD_i <= 1e-13 (flat window)      -> inorm NaN        (cpu_stats.cpp:146)
+//   inorm_i = 1/sqrt(SSD_i)                               (cpu_stats.cpp:150)
+//   df_i = (T[i+m]-T[i])/2, dg_i = (T[i+m]-mu[i+1])+(T[i]-mu[i]), last = 0
+//                                                         (cpu_stats.cpp:154-157)
+#include "device_common.h"
+#include "kernels.h"
+
+namespace scamp_amd {
+namespace {
+
+constexpr int kStatBlock = 256;
+constexpr int kStatChunk = 1024;
+constexpr double kFlatness = 1e-13;
+
+__global__ void clean_kernel(const double *__restrict__ T, int64_t n,
+                             double *__restrict__ clean, int64_t clean_len) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= clean_len) return;
+  double v = i < n ? T[i] : 0.0;
+  clean[i] = isfinite(v) ? v : 0.0;
+}
+
+// One thread per window.  The block's samples are staged through LDS in
+// chunks (each sample is read by up to m windows); pass 1 sums the window
+// with an error-free two-sum accumulation and flags non-finite samples, pass
+// 2 accumulates the squared deviations from the mean.
+__global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
+    const double *__restrict__ T, int64_t n, int64_t m,
+    double *__restrict__ means, double *__restrict__ inorms, int64_t ns,
+    int64_t stat_len) {
+  __shared__ double tile[kStatBlock + kStatChunk];
+  const int64_t w0 = blockIdx.x * (int64_t)kStatBlock;
+  const int64_t w = w0 + threadIdx.x;
+  double hi = 0, lo = 0;
+  bool bad = false;
+  for (int64_t t0 = 0; t0 < m; t0 += kStatChunk) {
+    const int64_t span = kStatBlock + kStatChunk;
+    __syncthreads();
+    for (int64_t j = threadIdx.x; j < span; j += kStatBlock) {
+      int64_t g = w0 + t0 + j;
+      tile[j] = g < n ? T[g] : 0.0;
+    }
+    __syncthreads();
+    const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
+    if (w < ns) {
+      for (int64_t t = 0; t < tend; ++t) {
+        double v = tile[threadIdx.x + t];
+        if (!isfinite(v)) {
+          bad = true;
+          v = 0.0;
+        }
+        double s = hi + v;
+        double z = s - hi;
+        lo += (hi - (s - z)) + (v - z);
+        hi = s;
+      }
+    }
+  }
+  const double mu = (hi + lo) / (double)m;
+  double ssd = 0;
+  for (int64_t t0 = 0; t0 < m; t0 += kStatChunk) {
+    const int64_t span = kStatBlock + kStatChunk;
+    __syncthreads();
+    for (int64_t j = threadIdx.x; j < span; j += kStatBlock) {
+      int64_t g = w0 + t0 + j;
+      double v = g < n ? T[g] : 0.0;
+      tile[j] = isfinite(v) ? v : 0.0;
+    }
+    __syncthreads();
+    const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
+    if (w < ns) {
+      for (int64_t t = 0; t < tend; ++t) {
+        double d = tile[threadIdx.x + t] - mu;
+        ssd = fma(d, d, ssd);
+      }
+    }
+  }
+  if (w < ns) {
+    means[w] = mu;
+    inorms[w] = (bad || ssd <= kFlatness) ? NAN : 1.0 / sqrt(ssd);
+  } else if (w < stat_len) {
+    means[w] = 0.0;
+    inorms[w] = NAN;
+  }
+}
+
+__global__ void dfdg_kernel(const double *__restrict__ clean, int64_t m,
+                            const double *__restrict__ means, int64_t ns,
+                            double *__restrict__ df, double *__restrict__ dg,
+                            int64_t stat_len) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= stat_len) return;
+  if (i < ns - 1) {
+    double a = clean[i], b = clean[i + m];
+    df[i] = (b - a) / 2.0;
+    dg[i] = (b - means[i + 1]) + (a - means[i]);
+  } else {
+    df[i] = 0.0;
+    dg[i] = 0.0;
+  }
+}
+
+__global__ void to_f32_kernel(const double *__restrict__ src,
+                              float *__restrict__ dst, int64_t len) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) dst[i] = (float)src[i];
+}
+
+// Exact-summation seeds: Ogita-Rump-Oishi Dot2 (TwoProduct via FMA +
+// TwoSum), the summation the reference's MATLAB notes describe
+// (test/dot2s.m); accurate to about one rounding of the true sum of the
+// centred products.  Thread j of a task owns diagonal diag0 + j; the row
+// window Y is shared by the task (uniform loads), X is read coalesced.
+__global__ __launch_bounds__(256) void seed_kernel(
+    const SeedTask *__restrict__ tasks, const double *__restrict__ X,
+    const double *__restrict__ muX, const double *__restrict__ Y,
+    const double *__restrict__ muY, int64_t m, double *__restrict__ out) {
+  const SeedTask task = tasks[blockIdx.x];
+  const int j = threadIdx.x;
+  if (j >= task.count) return;
+  const int64_t col = task.row + task.diag0 + j;
+  const double mx = muX[col];
+  const double my = muY[task.row];
+  const double *x = X + col;
+  const double *y = Y + task.row;
+  double s = 0, c = 0;
+  for (int64_t t = 0; t < m; ++t) {
+    const double a = y[t] - my;
+    const double b = x[t] - mx;
+    const double p = a * b;
+    const double e = fma(a, b, -p);
+    const double sn = s + p;
+    const double z = sn - s;
+    c += (s - (sn - z)) + (p - z) + e;
+    s = sn;
+  }
+  out[task.out_offset + j] = s + c;
+}
+
+__global__ void mp_to_keys_kernel(uint64_t *p, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t e = p[i];
+  float corr = bits_f32(static_cast<uint32_t>(e));
+  uint32_t idx = static_cast<uint32_t>(e >> 32);
+  p[i] = make_key(corr, idx);
+}
+
+__global__ void keys_to_mp_kernel(uint64_t *p, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = p[i];
+  p[i] = (static_cast<uint64_t>(key_index(k)) << 32) | f32_bits(key_corr(k));
+}
+
+__global__ void f32_to_keys_kernel(const float *src, uint64_t *dst, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = make_key(src[i], 0xFFFFFFFFu);
+}
+
+__global__ void keys_to_f32_kernel(const uint64_t *src, float *dst, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = key_corr(src[i]);
+}
+
+__global__ void f32_to_enc_kernel(float *p, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) reinterpret_cast<uint32_t *>(p)[i] = enc_corr(p[i]);
+}
+
+__global__ void enc_to_f32_kernel(float *p, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = dec_corr(reinterpret_cast<uint32_t *>(p)[i]);
+}
+
+__global__ void fill_u64_kernel(uint64_t *p, int64_t n, uint64_t v) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline unsigned blocks_for(int64_t n, int bs) {
+  return static_cast<unsigned>((n + bs - 1) / bs);
+}
+
+}  // namespace
+
+hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
+                        int64_t clean_len, double *means, double *inorms,
+                        double *df, double *dg, int64_t stat_len,
+                        hipStream_t s) {
+  const int64_t ns = n - m + 1;
+  hipLaunchKernelGGL(clean_kernel, dim3(blocks_for(clean_len, 256)), dim3(256),
+                     0, s, T, n, clean, clean_len);
+  hipLaunchKernelGGL(window_stats_kernel,
+                     dim3(blocks_for(stat_len, kStatBlock)), dim3(kStatBlock),
+                     0, s, T, n, m, means, inorms, ns, stat_len);
+  hipLaunchKernelGGL(dfdg_kernel, dim3(blocks_for(stat_len, 256)), dim3(256),
+                     0, s, clean, m, means, ns, df, dg, stat_len);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double *src, float *dst, int64_t len,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(len, 256)), dim3(256), 0,
+                     s, src, dst, len);
+  return hipGetLastError();
+}
+
+hipError_t launch_seeds(const SeedTask *tasks, int64_t ntasks,
+                        const double *X, const double *muX, const double *Y,
+                        const double *muY, int64_t m, double *out,
+                        hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(seed_kernel, dim3(static_cast<unsigned>(ntasks)),
+                     dim3(256), 0, s, tasks, X, muX, Y, muY, m, out);
+  return hipGetLastError();
+}
+
+#define SCAMP_SIMPLE_LAUNCH(kernel, n, ...)                                   \
+  do {                                                                        \
+    if ((n) > 0)                                                              \
+      hipLaunchKernelGGL(kernel, dim3(blocks_for((n), 256)), dim3(256), 0, s, \
+                         __VA_ARGS__);                                        \
+    return hipGetLastError();                                                 \
+  } while (0)
+
+hipError_t launch_mp_to_keys(uint64_t *p, int64_t n, hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(mp_to_keys_kernel, n, p, n);
+}
+hipError_t launch_keys_to_mp(uint64_t *p, int64_t n, hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(keys_to_mp_kernel, n, p, n);
+}
+hipError_t launch_f32_to_keys(const float *src, uint64_t *dst, int64_t n,
+                              hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(f32_to_keys_kernel, n, src, dst, n);
+}
+hipError_t launch_keys_to_f32(const uint64_t *src, float *dst, int64_t n,
+                              hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(keys_to_f32_kernel, n, src, dst, n);
+}
+hipError_t launch_f32_to_enc(float *p, int64_t n, hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(f32_to_enc_kernel, n, p, n);
+}
+hipError_t launch_enc_to_f32(float *p, int64_t n, hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(enc_to_f32_kernel, n, p, n);
+}
+hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(fill_u64_kernel, n, p, n, v);
+}
+
+}  // namespace scamp_amd

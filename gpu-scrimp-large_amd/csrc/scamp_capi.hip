@@ -1,0 +1,614 @@
+// C ABI of the engine (include/scamp_hip.h): argument validation, device
+// buffers, the statistics / seed / sweep launches of one join, and the
+// conversion of profiles between SCAMP's boundary layouts and the device keys.
+//
+// One join replaces SCAMP_Operation::do_join (src/SCAMP.cpp:174-259): where
+// the reference precomputes statistics on one host thread, cuts the distance
+// matrix into tiles, seeds each tile with cuFFT and merges each tile's
+// profile on the host under a mutex, this runs the statistics on the GPU,
+// seeds every wave band exactly, sweeps the whole triangle (self-join) or both
+// triangles (AB-join) with one persistent kernel per diagonal range, and
+// merges in place with atomics; the profile stays in HBM until the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/scamp_hip.h"
+#include "device_common.h"
+#include "kernels.h"
+
+using namespace scamp_amd;
+
+struct scamp_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  float total_ms = 0, prep_ms = 0, sweep_ms = 0;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+struct HipFailure {
+  hipError_t err;
+  const char *what;
+};
+
+#define HIP_CHECK(call)                                    \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) throw HipFailure{e_, #call};     \
+  } while (0)
+
+// Device allocations owned by one join; freed on scope exit.
+struct DevArena {
+  std::vector<void *> ptrs;
+  template <typename T>
+  T *alloc(int64_t count) {
+    void *p = nullptr;
+    size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+    HIP_CHECK(hipMalloc(&p, bytes));
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+  ~DevArena() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+// Statistics of one series, in the padded device layout the sweep reads.
+struct SeriesStats {
+  int64_t n = 0, ns = 0;
+  double *clean = nullptr, *mean = nullptr, *inorm = nullptr, *df = nullptr,
+         *dg = nullptr;
+  float *f_inorm = nullptr, *f_df = nullptr, *f_dg = nullptr;
+};
+
+SeriesStats make_stats(DevArena &ar, const double *d_T, int64_t n, int64_t m,
+                       bool fp32, hipStream_t s) {
+  SeriesStats st;
+  st.n = n;
+  st.ns = n - m + 1;
+  const int64_t clean_len = n + kColumnPad;
+  const int64_t stat_len = st.ns + kColumnPad;
+  st.clean = ar.alloc<double>(clean_len);
+  st.mean = ar.alloc<double>(stat_len);
+  st.inorm = ar.alloc<double>(stat_len);
+  st.df = ar.alloc<double>(stat_len);
+  st.dg = ar.alloc<double>(stat_len);
+  HIP_CHECK(launch_stats(d_T, n, m, st.clean, clean_len, st.mean, st.inorm,
+                         st.df, st.dg, stat_len, s));
+  if (fp32) {
+    st.f_inorm = ar.alloc<float>(stat_len);
+    st.f_df = ar.alloc<float>(stat_len);
+    st.f_dg = ar.alloc<float>(stat_len);
+    HIP_CHECK(launch_to_f32(st.inorm, st.f_inorm, stat_len, s));
+    HIP_CHECK(launch_to_f32(st.df, st.f_df, stat_len, s));
+    HIP_CHECK(launch_to_f32(st.dg, st.f_dg, stat_len, s));
+  }
+  return st;
+}
+
+// [lo, hi) minus the open interval (xlo, xhi): at most two ranges.
+std::vector<std::pair<int64_t, int64_t>> cut(int64_t lo, int64_t hi,
+                                             int64_t xlo, int64_t xhi) {
+  std::vector<std::pair<int64_t, int64_t>> out;
+  if (lo >= hi) return out;
+  // excluded integers: xlo < d < xhi
+  int64_t ex0 = xlo + 1, ex1 = xhi;  // [ex0, ex1)
+  if (ex0 >= ex1 || ex1 <= lo || ex0 >= hi) {
+    out.emplace_back(lo, hi);
+    return out;
+  }
+  if (lo < ex0) out.emplace_back(lo, ex0);
+  if (ex1 < hi) out.emplace_back(ex1, hi);
+  return out;
+}
+
+uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+struct Pass {
+  const SeriesStats *X, *Y;
+  int64_t d_lo, d_hi;
+  bool cols, rows;
+  uint64_t *keyX, *keyY;
+  double *sumX, *sumY;
+};
+
+// Runs one pass (one diagonal range of one orientation).
+void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
+              bool fp32, SweepKind kind, uint32_t *mat, int64_t mat_w,
+              int64_t rpc, int64_t cpc, double thr, int64_t anchor,
+              float *sweep_ms) {
+  hipStream_t s = ctx->stream;
+  const int64_t nX = p.X->ns, nY = p.Y->ns;
+  const int64_t ndiag = p.d_hi - p.d_lo;
+  if (ndiag <= 0 || (!p.cols && !p.rows)) return;
+  const int64_t nbands = (ndiag + kBand - 1) / kBand;
+
+  // Rotation rows of the bands and the seed tasks.
+  std::vector<int64_t> r0(nbands);
+  std::vector<SeedTask> tasks;
+  for (int64_t b = 0; b < nbands; ++b) {
+    const int64_t D0 = p.d_lo + b * kBand;
+    const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
+    const int64_t rfull = std::min(nY, nX - dlast);
+    r0[b] = rfull > 1 ? (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
+                                  (uint64_t)rfull)
+                      : 0;
+    if (!fp32) {
+      const int64_t cnt = dlast - D0 + 1;
+      for (int64_t j = 0; j < cnt; j += 256)
+        tasks.push_back(SeedTask{r0[b], D0 + j, b * kBand + j,
+                                 (int32_t)std::min<int64_t>(256, cnt - j), 0});
+    }
+  }
+  int64_t *d_r0 = ar.alloc<int64_t>(nbands);
+  HIP_CHECK(hipMemcpyAsync(d_r0, r0.data(), sizeof(int64_t) * nbands,
+                           hipMemcpyHostToDevice, s));
+  double *seedB = nullptr, *seed0 = nullptr;
+  if (!fp32) {
+    const int64_t nb_tasks = (int64_t)tasks.size();
+    for (int64_t d = 0; d < ndiag; d += 256)
+      tasks.push_back(SeedTask{0, p.d_lo + d, nbands * kBand + d,
+                               (int32_t)std::min<int64_t>(256, ndiag - d), 0});
+    (void)nb_tasks;
+    // seedB (nbands*kBand) followed by seed0 (ndiag + band padding)
+    double *seeds = ar.alloc<double>(nbands * kBand + ndiag + kBand);
+    SeedTask *d_tasks = ar.alloc<SeedTask>((int64_t)tasks.size());
+    HIP_CHECK(hipMemcpyAsync(d_tasks, tasks.data(),
+                             sizeof(SeedTask) * tasks.size(),
+                             hipMemcpyHostToDevice, s));
+    HIP_CHECK(launch_seeds(d_tasks, (int64_t)tasks.size(), p.X->clean,
+                           p.X->mean, p.Y->clean, p.Y->mean, m, seeds, s));
+    seedB = seeds;
+    seed0 = seeds + nbands * kBand;
+  }
+
+  unsigned int *counter = ar.alloc<unsigned int>(1);
+  HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
+
+  SweepArgs a{};
+  if (fp32) {
+    a.dfX = p.X->f_df; a.dgX = p.X->f_dg; a.inX = p.X->f_inorm;
+    a.dfY = p.Y->f_df; a.dgY = p.Y->f_dg; a.inY = p.Y->f_inorm;
+  } else {
+    a.dfX = p.X->df; a.dgX = p.X->dg; a.inX = p.X->inorm;
+    a.dfY = p.Y->df; a.dgY = p.Y->dg; a.inY = p.Y->inorm;
+  }
+  a.TX = p.X->clean; a.TY = p.Y->clean; a.muX = p.X->mean; a.muY = p.Y->mean;
+  a.seed0 = seed0; a.seedB = seedB; a.band_r0 = d_r0;
+  a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi;
+  a.nbands = nbands; a.m = m; a.anchor = anchor;
+  a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
+  a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
+  a.counter = counter;
+
+  const int grid = (int)std::min<int64_t>(nbands, (int64_t)ctx->num_cus * 16);
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, s));
+  HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s));
+  HIP_CHECK(hipEventRecord(e1, s));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *sweep_ms += ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+const char *validate(const scamp_hip_params *p, int64_t na, int64_t nb) {
+  // SCAMPArgs::validate (src/common.cpp:417-436)
+  if (p->window < 3) return "Error: Subsequence length must be at least 3";
+  if (p->max_tile_size < 1024)
+    return "Error: max tile size must be at least 1024";
+  if (p->max_tile_size / 2 < p->window)
+    return "Error: Tile length and width must be at least 2x larger than the "
+           "window size";
+  if (na < p->window || (p->has_b && nb < p->window))
+    return "Error: Input time series must be at least 'subesequence window "
+           "size' in length";
+  return nullptr;
+}
+
+int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
+                const double *d_b, int64_t nb, const scamp_hip_params *p,
+                void *d_prof_a, void *d_prof_b) {
+  const int64_t m = p->window;
+  const bool self = !p->has_b;
+  if (self) {
+    d_b = d_a;
+    nb = na;
+  }
+  const int pt = p->profile_type;
+  if (pt != SCAMP_HIP_1NN_INDEX && pt != SCAMP_HIP_1NN &&
+      pt != SCAMP_HIP_SUM_THRESH && pt != SCAMP_HIP_MATRIX_SUMMARY)
+    return fail(SCAMP_HIP_UNIMPLEMENTED,
+                "profile type not supported by this engine");
+  int prec = p->precision;
+  if (prec != SCAMP_HIP_SINGLE && prec != SCAMP_HIP_MIXED &&
+      prec != SCAMP_HIP_DOUBLE && prec != SCAMP_HIP_ULTRA)
+    return fail(SCAMP_HIP_INVALID_ARGS, "invalid precision type");
+  const bool fp32 = prec == SCAMP_HIP_MIXED || prec == SCAMP_HIP_SINGLE;
+  if (pt == SCAMP_HIP_MATRIX_SUMMARY &&
+      (p->matrix_height <= 0 || p->matrix_width <= 0))
+    return fail(SCAMP_HIP_INVALID_ARGS, "matrix height and width must be > 0");
+  int64_t anchor = p->anchor_interval > 0 ? p->anchor_interval : 2048;
+  anchor = std::max<int64_t>(16, (anchor + 15) / 16 * 16);
+
+  hipStream_t s = ctx->stream;
+  DevArena ar;
+  hipEvent_t ev_start, ev_prep, ev_end;
+  HIP_CHECK(hipEventCreate(&ev_start));
+  HIP_CHECK(hipEventCreate(&ev_prep));
+  HIP_CHECK(hipEventCreate(&ev_end));
+  HIP_CHECK(hipEventRecord(ev_start, s));
+
+  SeriesStats A = make_stats(ar, d_a, na, m, fp32, s);
+  SeriesStats B = self ? A : make_stats(ar, d_b, nb, m, fp32, s);
+  const int64_t nsA = A.ns, nsB = B.ns;
+  const bool rows_out = p->computing_rows && (self || p->keep_rows_separate);
+  const bool cols_out = p->computing_columns != 0;
+  const bool sep_rows = p->computing_rows && p->keep_rows_separate;
+
+  // Working profiles (padded) initialised from the caller's prior.
+  uint64_t *keyA = nullptr, *keyB = nullptr;
+  double *sumA = nullptr, *sumB = nullptr;
+  uint32_t *mat = nullptr;
+  const int64_t H = p->matrix_height, W = p->matrix_width;
+  if (pt == SCAMP_HIP_1NN_INDEX || pt == SCAMP_HIP_1NN) {
+    const uint64_t init = make_key(-2.0f, 0xFFFFFFFFu);
+    keyA = ar.alloc<uint64_t>(nsA + kColumnPad);
+    HIP_CHECK(launch_fill_u64(keyA + nsA, kColumnPad, init, s));
+    if (pt == SCAMP_HIP_1NN_INDEX) {
+      HIP_CHECK(hipMemcpyAsync(keyA, d_prof_a, sizeof(uint64_t) * nsA,
+                               hipMemcpyDeviceToDevice, s));
+      HIP_CHECK(launch_mp_to_keys(keyA, nsA, s));
+    } else {
+      HIP_CHECK(launch_f32_to_keys((const float *)d_prof_a, keyA, nsA, s));
+    }
+    if (sep_rows) {
+      keyB = ar.alloc<uint64_t>(nsB + kColumnPad);
+      HIP_CHECK(launch_fill_u64(keyB + nsB, kColumnPad, init, s));
+      if (pt == SCAMP_HIP_1NN_INDEX) {
+        HIP_CHECK(hipMemcpyAsync(keyB, d_prof_b, sizeof(uint64_t) * nsB,
+                                 hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(launch_mp_to_keys(keyB, nsB, s));
+      } else {
+        HIP_CHECK(launch_f32_to_keys((const float *)d_prof_b, keyB, nsB, s));
+      }
+    }
+  } else if (pt == SCAMP_HIP_SUM_THRESH) {
+    // SCAMP adds tile sums into the caller's profile (src/common.cpp:167-173)
+    sumA = static_cast<double *>(d_prof_a);
+    if (sep_rows) sumB = static_cast<double *>(d_prof_b);
+  } else {
+    mat = ar.alloc<uint32_t>(H * W);
+    HIP_CHECK(hipMemcpyAsync(mat, d_prof_a, sizeof(float) * H * W,
+                             hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(launch_f32_to_enc(reinterpret_cast<float *>(mat), H * W, s));
+  }
+  HIP_CHECK(hipEventRecord(ev_prep, s));
+
+  // Passes.  Self-join: diagonals [m/4, nsA) of the upper triangle, each pair
+  // updating its row and its column in the same profile (src/tile.cpp:12-38
+  // keeps |i - j| < m/4 out).  AB-join: the upper triangle (columns = A,
+  // rows = B, diagonals >= 0) and the lower one transposed (columns = B,
+  // rows = A, diagonals >= 1), as do_ab_join_full does per tile
+  // (src/tile.cpp:645-707); aligned AB-joins drop |global col - global row| <
+  // m/4 (src/tile.cpp:44-138).
+  const int64_t e = m / 4;
+  SweepKind kind = kNN;
+  int64_t rpc = 1, cpc = 1;
+  if (pt == SCAMP_HIP_SUM_THRESH) kind = kSum;
+  if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
+    cpc = (nsA + W - 1) / W;
+    rpc = (nsB + H - 1) / H;
+    kind = (cpc >= kDiagsPerLane && rpc >= kDiagsPerLane) ? kMatrix
+                                                          : kMatrixSmall;
+  }
+  std::vector<Pass> passes;
+  if (self) {
+    for (auto &r : cut(e, nsA, 0, 0)) {
+      Pass ps{&A, &A, r.first, r.second, cols_out, rows_out, keyA,
+              sep_rows ? keyB : keyA, sumA, sep_rows ? sumB : sumA};
+      if (pt == SCAMP_HIP_MATRIX_SUMMARY) ps.cols = ps.rows = true;
+      passes.push_back(ps);
+    }
+  } else {
+    int64_t delta = 0;
+    if (p->distributed_start_col >= 0 && p->distributed_start_row >= 0)
+      delta = p->distributed_start_col - p->distributed_start_row;
+    const bool al = p->is_aligned != 0 && e > 0;
+    // upper: d = j - i; excluded when |d + delta| < e
+    for (auto &r : al ? cut(0, nsA, -e - delta, e - delta)
+                      : cut(0, nsA, 0, 0)) {
+      Pass ps{&A, &B, r.first, r.second, cols_out, sep_rows, keyA, keyB,
+              sumA, sumB};
+      if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
+        ps.cols = true;
+        ps.rows = false;
+      }
+      passes.push_back(ps);
+    }
+    // lower (transposed): d = i - j >= 1; excluded when |d - delta| < e
+    for (auto &r : al ? cut(1, nsB, delta - e, delta + e)
+                      : cut(1, nsB, 0, 0)) {
+      Pass ps{&B, &A, r.first, r.second, sep_rows, cols_out, keyB, keyA,
+              sumB, sumA};
+      if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
+        ps.cols = false;
+        ps.rows = true;
+      }
+      passes.push_back(ps);
+    }
+  }
+  float sweep_ms = 0;
+  for (const Pass &ps : passes)
+    run_pass(ctx, ar, ps, m, fp32, kind, mat, W, rpc, cpc,
+             p->distance_threshold, anchor, &sweep_ms);
+
+  // Back to SCAMP's layouts.
+  if (pt == SCAMP_HIP_1NN_INDEX) {
+    HIP_CHECK(launch_keys_to_mp(keyA, nsA, s));
+    HIP_CHECK(hipMemcpyAsync(d_prof_a, keyA, sizeof(uint64_t) * nsA,
+                             hipMemcpyDeviceToDevice, s));
+    if (sep_rows) {
+      HIP_CHECK(launch_keys_to_mp(keyB, nsB, s));
+      HIP_CHECK(hipMemcpyAsync(d_prof_b, keyB, sizeof(uint64_t) * nsB,
+                               hipMemcpyDeviceToDevice, s));
+    }
+  } else if (pt == SCAMP_HIP_1NN) {
+    HIP_CHECK(launch_keys_to_f32(keyA, (float *)d_prof_a, nsA, s));
+    if (sep_rows) HIP_CHECK(launch_keys_to_f32(keyB, (float *)d_prof_b, nsB, s));
+  } else if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
+    HIP_CHECK(launch_enc_to_f32(reinterpret_cast<float *>(mat), H * W, s));
+    HIP_CHECK(hipMemcpyAsync(d_prof_a, mat, sizeof(float) * H * W,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  HIP_CHECK(hipEventRecord(ev_end, s));
+  HIP_CHECK(hipEventSynchronize(ev_end));
+  float total = 0;
+  HIP_CHECK(hipEventElapsedTime(&total, ev_start, ev_end));
+  ctx->total_ms = total;
+  ctx->sweep_ms = sweep_ms;
+  ctx->prep_ms = total - sweep_ms;
+  (void)hipEventDestroy(ev_start);
+  (void)hipEventDestroy(ev_prep);
+  (void)hipEventDestroy(ev_end);
+  if (!p->silent_mode)
+    std::printf("scamp_hip: join of %lld x %lld subsequences in %.3f ms "
+                "(sweep %.3f ms)\n",
+                (long long)nsA, (long long)nsB, total, sweep_ms);
+  return SCAMP_HIP_OK;
+}
+
+size_t profile_elem_bytes(int pt) {
+  switch (pt) {
+    case SCAMP_HIP_1NN_INDEX: return sizeof(uint64_t);
+    case SCAMP_HIP_SUM_THRESH: return sizeof(double);
+    default: return sizeof(float);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int scamp_hip_device_count(int *count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  if (count) *count = n;
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_ctx_create(int device, scamp_hip_ctx **out) {
+  if (!out) return fail(SCAMP_HIP_INVALID_ARGS, "null output pointer");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(SCAMP_HIP_NO_DEVICE, "no HIP device is visible");
+  if (device < 0 || device >= n)
+    return fail(SCAMP_HIP_INVALID_ARGS, "device id out of range");
+  try {
+    auto *ctx = new scamp_hip_ctx();
+    ctx->device = device;
+    HIP_CHECK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    ctx->num_cus = prop.multiProcessorCount;
+    HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx;
+  } catch (const HipFailure &f) {
+    return fail(SCAMP_HIP_DEVICE_ERROR,
+                std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx) {
+  if (!ctx) return SCAMP_HIP_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_last_error(char *buf, size_t len) {
+  if (buf && len) {
+    std::strncpy(buf, g_last_error.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
+                          const double *d_b, size_t nb,
+                          const scamp_hip_params *p, void *d_profile_a,
+                          void *d_profile_b) {
+  if (!ctx || !p || !d_a || !d_profile_a)
+    return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  if (p->has_b && !d_b) return fail(SCAMP_HIP_INVALID_ARGS, "missing series b");
+  if (const char *msg = validate(p, (int64_t)na, (int64_t)nb))
+    return fail(SCAMP_HIP_INVALID_ARGS, msg);
+  if (p->computing_rows && p->keep_rows_separate && !d_profile_b &&
+      p->profile_type != SCAMP_HIP_MATRIX_SUMMARY)
+    return fail(SCAMP_HIP_INVALID_ARGS, "missing profile b");
+  try {
+    HIP_CHECK(hipSetDevice(ctx->device));
+    return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, p, d_profile_a,
+                       d_profile_b);
+  } catch (const HipFailure &f) {
+    (void)hipStreamSynchronize(ctx->stream);
+    int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
+                                            : SCAMP_HIP_DEVICE_ERROR;
+    return fail(code, std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+}
+
+int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
+                   const double *b, size_t nb, const scamp_hip_params *p,
+                   void *profile_a, void *profile_b) {
+  if (!ctx || !p || !a || !profile_a)
+    return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  if (p->has_b && !b) return fail(SCAMP_HIP_INVALID_ARGS, "missing series b");
+  if (const char *msg = validate(p, (int64_t)na, (int64_t)nb))
+    return fail(SCAMP_HIP_INVALID_ARGS, msg);
+  const int64_t m = p->window;
+  const size_t eb = profile_elem_bytes(p->profile_type);
+  const bool mat = p->profile_type == SCAMP_HIP_MATRIX_SUMMARY;
+  const size_t len_a =
+      mat ? (size_t)(p->matrix_height * p->matrix_width) : na - m + 1;
+  const bool sep = !mat && p->computing_rows && p->keep_rows_separate;
+  const size_t len_b = sep ? (p->has_b ? nb : na) - m + 1 : 0;
+  if (sep && !profile_b) return fail(SCAMP_HIP_INVALID_ARGS, "missing profile b");
+  try {
+    HIP_CHECK(hipSetDevice(ctx->device));
+    DevArena ar;
+    hipStream_t s = ctx->stream;
+    double *d_a = ar.alloc<double>((int64_t)na);
+    HIP_CHECK(hipMemcpyAsync(d_a, a, sizeof(double) * na, hipMemcpyHostToDevice, s));
+    double *d_b = nullptr;
+    if (p->has_b) {
+      d_b = ar.alloc<double>((int64_t)nb);
+      HIP_CHECK(hipMemcpyAsync(d_b, b, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    }
+    void *d_pa = ar.alloc<char>((int64_t)(eb * len_a));
+    HIP_CHECK(hipMemcpyAsync(d_pa, profile_a, eb * len_a, hipMemcpyHostToDevice, s));
+    void *d_pb = nullptr;
+    if (sep) {
+      d_pb = ar.alloc<char>((int64_t)(eb * len_b));
+      HIP_CHECK(hipMemcpyAsync(d_pb, profile_b, eb * len_b, hipMemcpyHostToDevice, s));
+    }
+    int rc = join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, p, d_pa, d_pb);
+    if (rc != SCAMP_HIP_OK) return rc;
+    HIP_CHECK(hipMemcpyAsync(profile_a, d_pa, eb * len_a, hipMemcpyDeviceToHost, s));
+    if (sep)
+      HIP_CHECK(hipMemcpyAsync(profile_b, d_pb, eb * len_b, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (const HipFailure &f) {
+    (void)hipStreamSynchronize(ctx->stream);
+    int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
+                                            : SCAMP_HIP_DEVICE_ERROR;
+    return fail(code, std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_stats(scamp_hip_ctx *ctx, const double *T, size_t n,
+                    int64_t window, double *means, double *inorms, double *df,
+                    double *dg) {
+  if (!ctx || !T || !means || !inorms || !df || !dg)
+    return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  if (window < 1 || (int64_t)n < window)
+    return fail(SCAMP_HIP_INVALID_ARGS, "series shorter than the window");
+  try {
+    HIP_CHECK(hipSetDevice(ctx->device));
+    DevArena ar;
+    hipStream_t s = ctx->stream;
+    double *d_T = ar.alloc<double>((int64_t)n);
+    HIP_CHECK(hipMemcpyAsync(d_T, T, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    SeriesStats st = make_stats(ar, d_T, (int64_t)n, window, false, s);
+    const size_t bytes = sizeof(double) * st.ns;
+    HIP_CHECK(hipMemcpyAsync(means, st.mean, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(inorms, st.inorm, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(df, st.df, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(dg, st.dg, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (const HipFailure &f) {
+    return fail(SCAMP_HIP_DEVICE_ERROR,
+                std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
+                      const double *Y, size_t ny, int64_t window, int64_t row,
+                      int64_t diag0, int64_t count, double *qt) {
+  if (!ctx || !X || !Y || !qt) return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  const int64_t m = window;
+  if (m < 1 || (int64_t)nx < m || (int64_t)ny < m || row < 0 || diag0 < 0 ||
+      count < 0 || row + m > (int64_t)ny ||
+      row + diag0 + count - 1 + m > (int64_t)nx)
+    return fail(SCAMP_HIP_INVALID_ARGS, "seed window outside the series");
+  if (count == 0) return SCAMP_HIP_OK;
+  try {
+    HIP_CHECK(hipSetDevice(ctx->device));
+    DevArena ar;
+    hipStream_t s = ctx->stream;
+    double *dX = ar.alloc<double>((int64_t)nx);
+    double *dY = ar.alloc<double>((int64_t)ny);
+    HIP_CHECK(hipMemcpyAsync(dX, X, sizeof(double) * nx, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dY, Y, sizeof(double) * ny, hipMemcpyHostToDevice, s));
+    SeriesStats sx = make_stats(ar, dX, (int64_t)nx, m, false, s);
+    SeriesStats sy = make_stats(ar, dY, (int64_t)ny, m, false, s);
+    std::vector<SeedTask> tasks;
+    for (int64_t j = 0; j < count; j += 256)
+      tasks.push_back(SeedTask{row, diag0 + j, j,
+                               (int32_t)std::min<int64_t>(256, count - j), 0});
+    SeedTask *d_tasks = ar.alloc<SeedTask>((int64_t)tasks.size());
+    HIP_CHECK(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(SeedTask) * tasks.size(),
+                             hipMemcpyHostToDevice, s));
+    double *d_out = ar.alloc<double>(count);
+    HIP_CHECK(launch_seeds(d_tasks, (int64_t)tasks.size(), sx.clean, sx.mean,
+                           sy.clean, sy.mean, m, d_out, s));
+    HIP_CHECK(hipMemcpyAsync(qt, d_out, sizeof(double) * count,
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (const HipFailure &f) {
+    return fail(SCAMP_HIP_DEVICE_ERROR,
+                std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
+                          double *prep_ms, double *sweep_ms) {
+  if (!ctx) return fail(SCAMP_HIP_INVALID_ARGS, "null context");
+  if (total_ms) *total_ms = ctx->total_ms;
+  if (prep_ms) *prep_ms = ctx->prep_ms;
+  if (sweep_ms) *sweep_ms = ctx->sweep_ms;
+  return SCAMP_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+/*
+ * scamp_hip.h -- C ABI of the MI355X (gfx950) matrix-profile join engine.
+ *
+ * This is the boundary the host side (the C++ SCAMP::do_SCAMP entry point,
+ * the pyscamp module and the torch extension) calls.  Plain pointers and
+ * sizes only; every function returns an int status (scamp_hip_status) and
+ * never throws; the message of the last failure on the calling thread is
+ * available from scamp_hip_last_error().
+ *
+ * Profile buffers use SCAMP's layouts bit for bit (src/common.h:70-74,
+ * src/common.cpp:283-319):
+ *   1NN_INDEX      uint64 mp_entry: low 32 bits = FP32 Pearson correlation,
+ *                  high 32 bits = index; "no match" = {-2.0f, 0xFFFFFFFF}
+ *   1NN            float correlation, "no match" = -2
+ *   SUM_THRESH     double sum of correlations above the threshold, init 0
+ *   MATRIX_SUMMARY float[height * width] row-major, init -2
+ * Profiles are read as the prior best-so-far and updated in place, as
+ * SCAMP's tiles do (src/tile.cpp:322-389).
+ */
+#ifndef SCAMP_HIP_H
+#define SCAMP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  0..6 keep the values of SCAMPError_t (src/common.h:359-367);
+ * the rest are this library's. */
+typedef enum {
+  SCAMP_HIP_OK = 0,
+  SCAMP_HIP_UNIMPLEMENTED = 1,   /* SCAMP_FUNCTIONALITY_UNIMPLEMENTED */
+  SCAMP_HIP_ILLEGAL_TYPE = 2,    /* SCAMP_TILE_ILLEGAL_TYPE */
+  SCAMP_HIP_DEVICE_ERROR = 3,    /* SCAMP_CUDA_ERROR: a HIP runtime error */
+  SCAMP_HIP_DIM_INCOMPATIBLE = 6,/* SCAMP_DIM_INCOMPATIBLE */
+  SCAMP_HIP_INVALID_ARGS = 16,   /* argument validation failed */
+  SCAMP_HIP_OUT_OF_MEMORY = 17,
+  SCAMP_HIP_NO_DEVICE = 18
+} scamp_hip_status;
+
+/* Profile types: the SCAMPProfileType values (src/common.h:34-44). */
+enum {
+  SCAMP_HIP_1NN_INDEX = 1,
+  SCAMP_HIP_SUM_THRESH = 2,
+  SCAMP_HIP_1NN = 6,
+  SCAMP_HIP_MATRIX_SUMMARY = 8
+};
+
+/* Precisions: the SCAMPPrecisionType values (src/common.h:52-58).
+ * DOUBLE (and ULTRA, which maps to it) = all-FP64 recurrence;
+ * MIXED = FP32 recurrence re-anchored with FP64 every anchor_interval rows. */
+enum {
+  SCAMP_HIP_SINGLE = 1,
+  SCAMP_HIP_MIXED = 2,
+  SCAMP_HIP_DOUBLE = 3,
+  SCAMP_HIP_ULTRA = 4
+};
+
+typedef struct scamp_hip_ctx scamp_hip_ctx;
+
+/* Join options: the fields of SCAMPArgs (src/common.h:164-188) that select
+ * what is computed, plus the mixed form's re-anchoring interval. */
+typedef struct {
+  int64_t window;              /* m, >= 3 */
+  int32_t profile_type;        /* SCAMP_HIP_1NN_INDEX ... */
+  int32_t precision;           /* SCAMP_HIP_DOUBLE / _MIXED / ... */
+  int32_t has_b;               /* 0 = self-join of a */
+  int32_t computing_rows;      /* rows of the distance matrix (series b) */
+  int32_t computing_columns;   /* columns (series a) */
+  int32_t keep_rows_separate;  /* row profile goes to profile_b */
+  int32_t is_aligned;          /* AB-join of aligned series: apply m/4 zone */
+  int32_t silent_mode;
+  int64_t distributed_start_row;  /* -1 = unset */
+  int64_t distributed_start_col;
+  double distance_threshold;   /* SUM_THRESH / MATRIX_SUMMARY threshold */
+  int64_t matrix_height;
+  int64_t matrix_width;
+  int64_t max_tile_size;       /* validated as SCAMP does; the sweep is untiled */
+  int64_t anchor_interval;     /* MIXED only: rows between FP64 re-anchors,
+                                  0 = library default */
+} scamp_hip_params;
+
+/* Replaces: the per-worker device state of SCAMP's Tile/ExecInfo
+ * (src/tile.cpp:141-283, src/common.cpp:109-138).  Binds a HIP device and
+ * creates one stream on it. */
+int scamp_hip_ctx_create(int device, scamp_hip_ctx **out);
+int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx);
+
+/* Number of visible HIP devices (0 when none), as SCAMP's do_SCAMP counts
+ * CUDA devices (src/SCAMP.cpp:261-275). */
+int scamp_hip_device_count(int *count);
+
+/* Copies the last error message of the calling thread into buf. */
+int scamp_hip_last_error(char *buf, size_t len);
+
+/* The join.  Replaces SCAMP_Operation::do_join (src/SCAMP.cpp:174-259) with
+ * its tile queue, per-tile QT seeds (src/qt_helper.cpp:75-121), the
+ * gpu_kernel_* tile kernels (src/kernels.h:9-12) and the host merges
+ * (src/common.cpp:364-415).  a/b/profiles are HOST buffers; profile_b is
+ * used only with keep_rows_separate.  Blocks until the profiles are written. */
+int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
+                   const double *b, size_t nb, const scamp_hip_params *p,
+                   void *profile_a, void *profile_b);
+
+/* The same join on DEVICE buffers already resident in HBM (series as float64
+ * device pointers, profiles in the layouts above).  Work is queued on the
+ * context's stream; the call returns after the stream has finished. */
+int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
+                          const double *d_b, size_t nb,
+                          const scamp_hip_params *p, void *d_profile_a,
+                          void *d_profile_b);
+
+/* Statistics of one series on the GPU (replaces compute_statistics_cpu,
+ * src/cpu_stats.cpp:123-160, with direct compensated per-window sums).
+ * Host buffers: T[n]; outputs means/inorms/df/dg have n - m + 1 entries. */
+int scamp_hip_stats(scamp_hip_ctx *ctx, const double *T, size_t n,
+                    int64_t window, double *means, double *inorms, double *df,
+                    double *dg);
+
+/* Exact-summation FP64 QT seeds (replaces qt_compute_helper::compute_QT,
+ * src/qt_helper.cpp:75-121, and its cuFFT kernels, src/qt_kernels.cu:3-61):
+ *   qt[j] = sum_{t<m} (X[row+diag0+j+t] - mean(X window)) *
+ *                     (Y[row+t] - mean(Y window)),  j < count
+ * on host buffers X[nx], Y[ny] (NaN/inf read as 0, like the join). */
+int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
+                      const double *Y, size_t ny, int64_t window, int64_t row,
+                      int64_t diag0, int64_t count, double *qt);
+
+/* Timing of the last join on this context, from HIP events on its stream:
+ * total, statistics+seeds, and the diagonal sweep, in milliseconds. */
+int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
+                          double *prep_ms, double *sweep_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
