@@ -10,9 +10,10 @@ namespace scamp_amd {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kDiagsPerLane = 8;   // consecutive diagonals held by one lane
 constexpr int kBand = kWave * kDiagsPerLane;  // diagonals per wave band
-// Column-side arrays are padded so that the sweep's look-ahead loads and the
-// lanes of a partial last band never leave the allocation.
-constexpr int64_t kColumnPad = kBand + 8 * kDiagsPerLane;
+// Column-side arrays (statistics, profiles, clean series) are padded so that
+// the sweep's look-ahead loads, the lanes of a partial last band and the
+// FP32 form's re-anchoring windows never leave the allocation.
+constexpr int64_t kColumnPad = 2 * kBand + 8 * kDiagsPerLane;
 
 __host__ __device__ inline uint32_t f32_bits(float f) {
   uint32_t b;

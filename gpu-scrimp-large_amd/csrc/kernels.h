@@ -61,12 +61,14 @@ struct SweepArgs {
   const void *dfY, *dgY, *inY;
   // clean series and means (FP32 form: in-kernel FP64 re-anchoring)
   const double *TX, *TY, *muX, *muY;
-  // FP64 form: exact seeds at row 0 (indexed by diagonal - d_lo) and at each
-  // band's first row (kBand entries per band)
+  // FP64 form: exact seeds at row 0 (indexed by diagonal - d_base) and at
+  // each band's first row (kBand entries per band)
   const double *seed0, *seedB;
-  const int64_t *band_r0;
+  const int64_t *band_r0;  // first row of each band (multiple of 8)
   int64_t nX, nY;        // subsequence counts of X and Y
   int64_t d_lo, d_hi;    // diagonal range of this pass
+  int64_t d_base;        // d_lo rounded down to a multiple of 8: band b
+                         // covers diagonals d_base + 512 b + [0, 512)
   int64_t nbands;
   int64_t m;
   int64_t anchor;        // FP32 form: rows between re-anchors (multiple of 16)

@@ -142,22 +142,27 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   const int64_t nX = p.X->ns, nY = p.Y->ns;
   const int64_t ndiag = p.d_hi - p.d_lo;
   if (ndiag <= 0 || (!p.cols && !p.rows)) return;
-  const int64_t nbands = (ndiag + kBand - 1) / kBand;
+  // Bands start on multiples of kDiagsPerLane (the sweep indexes its column
+  // slots by column mod 8); diagonals below d_lo in the first band are NaN.
+  const int64_t d_base = p.d_lo - p.d_lo % kDiagsPerLane;
+  const int64_t nbands = (p.d_hi - d_base + kBand - 1) / kBand;
 
-  // Rotation rows of the bands and the seed tasks.
+  // First row of each band (a multiple of 8 below the band's full-row
+  // limit) and the seed tasks.
   std::vector<int64_t> r0(nbands);
   std::vector<SeedTask> tasks;
   for (int64_t b = 0; b < nbands; ++b) {
-    const int64_t D0 = p.d_lo + b * kBand;
+    const int64_t D0 = d_base + b * kBand;
+    const int64_t dfirst = std::max(D0, p.d_lo);
     const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
     const int64_t rfull = std::min(nY, nX - dlast);
-    r0[b] = rfull > 1 ? (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
-                                  (uint64_t)rfull)
-                      : 0;
+    const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
+    r0[b] = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
+                      (uint64_t)slots) * kDiagsPerLane;
     if (!fp32) {
-      const int64_t cnt = dlast - D0 + 1;
+      const int64_t cnt = dlast - dfirst + 1;
       for (int64_t j = 0; j < cnt; j += 256)
-        tasks.push_back(SeedTask{r0[b], D0 + j, b * kBand + j,
+        tasks.push_back(SeedTask{r0[b], dfirst + j, b * kBand + (dfirst - D0) + j,
                                  (int32_t)std::min<int64_t>(256, cnt - j), 0});
     }
   }
@@ -166,13 +171,12 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
                            hipMemcpyHostToDevice, s));
   double *seedB = nullptr, *seed0 = nullptr;
   if (!fp32) {
-    const int64_t nb_tasks = (int64_t)tasks.size();
+    // seed0 (row 0 of every diagonal, indexed by diagonal - d_base) after
+    // seedB (row r0 of each band)
     for (int64_t d = 0; d < ndiag; d += 256)
-      tasks.push_back(SeedTask{0, p.d_lo + d, nbands * kBand + d,
+      tasks.push_back(SeedTask{0, p.d_lo + d, nbands * kBand + (p.d_lo - d_base) + d,
                                (int32_t)std::min<int64_t>(256, ndiag - d), 0});
-    (void)nb_tasks;
-    // seedB (nbands*kBand) followed by seed0 (ndiag + band padding)
-    double *seeds = ar.alloc<double>(nbands * kBand + ndiag + kBand);
+    double *seeds = ar.alloc<double>(2 * nbands * kBand);
     SeedTask *d_tasks = ar.alloc<SeedTask>((int64_t)tasks.size());
     HIP_CHECK(hipMemcpyAsync(d_tasks, tasks.data(),
                              sizeof(SeedTask) * tasks.size(),
@@ -196,7 +200,7 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   }
   a.TX = p.X->clean; a.TY = p.Y->clean; a.muX = p.X->mean; a.muY = p.Y->mean;
   a.seed0 = seed0; a.seedB = seedB; a.band_r0 = d_r0;
-  a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi;
+  a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi; a.d_base = d_base;
   a.nbands = nbands; a.m = m; a.anchor = anchor;
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;

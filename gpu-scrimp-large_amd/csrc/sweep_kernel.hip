@@ -10,29 +10,34 @@
 // and the profile keeps, per column and per row, the best FP32 correlation and
 // its index (1NN_INDEX / 1NN), the sum of correlations above a threshold
 // (SUM_THRESH), or the max-pooled matrix (MATRIX_SUMMARY).  NaN correlations
-// (windows with a non-finite sample or flat windows) never match.
+// (windows with a non-finite sample, flat windows) never match.
 //
 // How it runs here (MI355X design, not the reference's):
 //  * Work unit = a wave band: 64 lanes x 8 consecutive diagonals = 512
 //    diagonals, swept down all their rows by one wavefront.  A persistent grid
-//    pulls bands from an atomic counter, longest first.
+//    pulls bands from an atomic counter, longest first.  Band origins are
+//    aligned to 8 diagonals (diagonals outside the pass carry NaN).
 //  * Each band starts at a pseudo-random row r0 (its FP64 seed computed
 //    exactly for that row) and wraps round to row 0, so the bands in flight
 //    sit at spread-out rows and each row's visits are spread over the run.
-//  * Per row a lane evaluates its 8 pairs from registers: the 8 covariances,
-//    a sliding window of 8 columns' update terms / inverse norms / thresholds
-//    (two register sets in ping-pong, refilled one column per row with loads
-//    issued a block of 8 rows before use) and the row's terms (uniform).
-//  * 1NN / 1NN_INDEX: instead of a max+argmax reduction for every pair, each
-//    correlation is tested against the current best of its row and of its
-//    column (kept in the profile itself, read as lowered thresholds).  Only
-//    when some lane passes does the wave take the slow path: exact FP32
+//  * A lane holds its 8 covariances and the terms (update terms, inverse
+//    norm, threshold) of the 8 columns its diagonals cross on the current
+//    row, in slots indexed by column mod 8.  Going down one row, every lane
+//    drops one column and needs one new one -- exactly the column its
+//    right-hand neighbour drops, in the same slot.  So one DPP wave shift of
+//    that slot per row moves every column one lane down; only lane 63 takes a
+//    new column, from an LDS stage filled a block of 8 rows ahead with
+//    coalesced loads.  Row terms come through the same stage.  Each column is
+//    read from memory once per band.
+//  * 1NN / 1NN_INDEX: each correlation is tested against the current best of
+//    its row and of its column (thresholds read from the profile itself).
+//    Only when some lane passes does the wave take the slow path: exact FP32
 //    rounding, lowest-index argmax within the wave, and 64-bit atomicMax of
 //    order-preserving keys straight into the profile.  The profile only grows,
-//    so stale thresholds only cost extra slow paths, never a missed match.
-//  * SUM_THRESH: row sums reduced across the wave per row; column partial
-//    sums carried lane to lane (a column leaves lane L into lane L-1) so each
-//    column reaches memory once per band.
+//    so a stale threshold costs an extra slow path, never a missed match.
+//  * SUM_THRESH: row sums reduced across the wave per row; a column's partial
+//    sum travels with the column in its slot and reaches memory once per band,
+//    when the column leaves lane 0.
 //  * MATRIX_SUMMARY: exact max-pooling; each lane keeps two running cell
 //    maxima (its 8 columns straddle at most one cell boundary) and flushes a
 //    cell with a 32-bit atomicMax when its columns or the row leave it.
@@ -43,18 +48,20 @@ namespace scamp_amd {
 namespace {
 
 constexpr int K = kDiagsPerLane;
+constexpr int kAnchorChunk = 256;  // u-samples per LDS stage of an anchor
 
 template <typename DT>
 struct Slot {
   DT df, dg, in;   // column update terms and inverse norm
-  DT t;            // 1NN: lowered threshold of the column's current best
-  uint32_t raw;    // 1NN: key high word as loaded, decoded one row later
-  double acc;      // SUM: this lane's partial column sum
+  DT t;            // 1NN: threshold of the column's current best
+  double acc;      // SUM: partial sum of the column, travelling with it
 };
 
+// Terms of one row or column, staged through LDS.
 template <typename DT>
-struct Cols {
-  Slot<DT> s[K];
+struct Terms {
+  DT df, dg, in;
+  DT t;  // 1NN: threshold of the row's / column's current best
 };
 
 template <typename DT>
@@ -68,6 +75,21 @@ __device__ __forceinline__ float nan_of<float>() {
   return __builtin_nanf("");
 }
 
+// Wavefront shift by one lane toward lane 0 (DPP wave_shl:1): lane L
+// receives lane L+1's value, lane 63 receives `edge`.
+__device__ __forceinline__ float shl1(float v, float edge) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      __float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double shl1(double v, double edge) {
+  const long long x = __double_as_longlong(v), e = __double_as_longlong(edge);
+  const int lo = __builtin_amdgcn_update_dpp((int)e, (int)x, 0x130, 0xf, 0xf,
+                                             false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(x >> 32),
+                                             0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Per-lane state of the MATRIX_SUMMARY kinds.
 struct CellAcc {
   double lo, hi;       // running maxima of the current cell and the next one
@@ -79,9 +101,10 @@ template <typename DT, int KIND, bool COLS, bool ROWS>
 struct Sweep {
   const SweepArgs &a;
   const int lane;
-  const int64_t D0;      // first diagonal of the band
+  const int64_t D0;      // first diagonal of the band (multiple of K)
   const int64_t d0;      // first diagonal of this lane
   DT cov[K];
+  Slot<DT> S[K];         // column slots, indexed by column mod K
   // MATRIX_SUMMARY state
   CellAcc dir, tra;
   int64_t dir_row_cell, dir_row_pos;  // uniform: row cell and row mod rpc
@@ -90,75 +113,138 @@ struct Sweep {
   __device__ Sweep(const SweepArgs &args, int ln, int64_t band_d0)
       : a(args), lane(ln), D0(band_d0), d0(band_d0 + (int64_t)ln * K) {}
 
-  __device__ __forceinline__ const DT *dfX() const { return (const DT *)a.dfX; }
-  __device__ __forceinline__ const DT *dgX() const { return (const DT *)a.dgX; }
-  __device__ __forceinline__ const DT *inX() const { return (const DT *)a.inX; }
-  __device__ __forceinline__ const DT *dfY() const { return (const DT *)a.dfY; }
-  __device__ __forceinline__ const DT *dgY() const { return (const DT *)a.dgY; }
-  __device__ __forceinline__ const DT *inY() const { return (const DT *)a.inY; }
+  __device__ __forceinline__ bool diag_valid(int64_t d) const {
+    return d >= a.d_lo && d < a.d_hi;
+  }
 
-  // Issue the loads of one column into a slot (the column index is always
-  // inside the padded arrays).
+  // Byte offsets into the padded arrays fit 32 bits, which lets every load
+  // use a uniform base address plus one per-lane 32-bit offset.
+  template <typename T>
+  __device__ __forceinline__ static T ld(const void *base, uint32_t byte_off) {
+    return *reinterpret_cast<const T *>(static_cast<const char *>(base) +
+                                        byte_off);
+  }
+
+  // Loads one column into a slot (segment start only; afterwards columns
+  // reach a lane from its right-hand neighbour).
   __device__ __forceinline__ void load_slot(Slot<DT> &s, int64_t col) const {
-    s.df = dfX()[col];
-    s.dg = dgX()[col];
-    s.in = inX()[col];
+    const uint32_t off = static_cast<uint32_t>(col) * sizeof(DT);
+    s.df = ld<DT>(a.dfX, off);
+    s.dg = ld<DT>(a.dgX, off);
+    s.in = ld<DT>(a.inX, off);
     if (KIND == kNN && COLS)
-      s.raw = reinterpret_cast<const uint32_t *>(a.keyX)[2 * col + 1];
+      s.t = Thresh<DT>::from_key_hi(
+          ld<uint32_t>(a.keyX, static_cast<uint32_t>(col) * 8u + 4u));
     if (KIND == kSum) s.acc = 0.0;
   }
-  __device__ __forceinline__ void decode_slot(Slot<DT> &s) const {
-    if (KIND == kNN && COLS) s.t = Thresh<DT>::from_key_hi(s.raw);
+
+  // Terms staged a block ahead: lane l fetches row ib+l (the rows of the
+  // block) and column ib+D0+kBand+l (the column entering the band at lane 63
+  // on row ib+l); lanes >= K fetch clamped / padded entries nobody reads, so
+  // the code has no divergent branch.
+  struct Pre {
+    DT rdf, rdg, rin;
+    uint32_t rkey;
+    DT cdf, cdg, cin;
+    uint32_t ckey;
+  };
+  __device__ __forceinline__ void prefetch(Pre &p, int64_t ib) const {
+    int64_t ri = ib + lane;
+    ri = ri < a.nY ? ri : a.nY - 1;
+    const uint32_t roff = static_cast<uint32_t>(ri) * sizeof(DT);
+    p.rdf = ld<DT>(a.dfY, roff);
+    p.rdg = ld<DT>(a.dgY, roff);
+    p.rin = ld<DT>(a.inY, roff);
+    if (KIND == kNN && ROWS)
+      p.rkey = ld<uint32_t>(a.keyY, static_cast<uint32_t>(ri) * 8u + 4u);
+    const int64_t ci = ib + D0 + kBand + lane;
+    const uint32_t coff = static_cast<uint32_t>(ci) * sizeof(DT);
+    p.cdf = ld<DT>(a.dfX, coff);
+    p.cdg = ld<DT>(a.dgX, coff);
+    p.cin = ld<DT>(a.inX, coff);
+    if (KIND == kNN && COLS)
+      p.ckey = ld<uint32_t>(a.keyX, static_cast<uint32_t>(ci) * 8u + 4u);
+  }
+  __device__ __forceinline__ void publish(const Pre &p, Terms<DT> *s_row,
+                                          Terms<DT> *s_col) const {
+    Terms<DT> r, c;
+    r.df = p.rdf;
+    r.dg = p.rdg;
+    r.in = p.rin;
+    r.t = (KIND == kNN && ROWS) ? Thresh<DT>::from_key_hi(p.rkey) : (DT)0;
+    c.df = p.cdf;
+    c.dg = p.cdg;
+    c.in = p.cin;
+    c.t = (KIND == kNN && COLS) ? Thresh<DT>::from_key_hi(p.ckey) : (DT)0;
+    s_row[lane] = r;
+    s_col[lane] = c;
+    __builtin_amdgcn_wave_barrier();
   }
 
   // ---------------------------------------------------------------- seeds
   // FP32 form: exact FP64 covariance of this lane's 8 diagonals at row i,
   //   cov_k = sum_t X[i+d_k+t]*yc_t - muX[i+d_k]*sum_t yc_t,
   //   yc_t = Y[i+t] - muY[i]
-  // swept as a sliding window: each loaded X sample feeds 8 products.
-  __device__ __forceinline__ void anchor(int64_t i) {
+  // The wave stages the X samples of its 512 diagonals and the centred Y
+  // window through LDS in chunks (coalesced loads, one pass), then each lane
+  // slides a register window over them so that each X sample feeds 8 FMAs.
+  __device__ __forceinline__ void anchor(int64_t i, double *s_ax,
+                                         double *s_ay) {
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    const double *x = a.TX + i + d0;
-    const double *y = a.TY + i;
     const double my = a.muY[i];
     const int64_t m = a.m;
+    const int64_t uend = m + K - 1;  // u = t + k runs over [0, m + K - 1)
+    const int xoff = lane * K;       // d0 - D0
     double sy = 0.0;
-    double yw[2 * K - 1];  // yc_t for t = u0-K+1 .. u0+K-1
+    double yw[2 * K - 1];            // yc_t for t = u0-K+1 .. u0+K-1
 #pragma unroll
     for (int q = 0; q < 2 * K - 1; ++q) yw[q] = 0.0;
-    for (int64_t u0 = 0; u0 < m + K - 1; u0 += K) {
-#pragma unroll
-      for (int q = 0; q < K - 1; ++q) yw[q] = yw[q + K];
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        const int64_t t = u0 + q;
-        const double v = t < m ? y[t] - my : 0.0;
-        yw[K - 1 + q] = v;
-        sy += v;
+    for (int64_t c0 = 0; c0 < uend; c0 += kAnchorChunk) {
+      __builtin_amdgcn_wave_barrier();
+      const double *xs = a.TX + i + D0 + c0;
+      for (int j = lane; j < kBand + kAnchorChunk; j += kWave) s_ax[j] = xs[j];
+      for (int j = lane; j < kAnchorChunk; j += kWave) {
+        const int64_t t = c0 + j;
+        s_ay[j] = t < m ? a.TY[i + t] - my : 0.0;
       }
-      double xv[K];
+      __builtin_amdgcn_wave_barrier();
+      const int64_t cend = uend - c0 < kAnchorChunk ? uend - c0 : kAnchorChunk;
+      for (int u = 0; u < cend; u += K) {
 #pragma unroll
-      for (int q = 0; q < K; ++q) xv[q] = x[u0 + q];
+        for (int q = 0; q < K - 1; ++q) yw[q] = yw[q + K];
 #pragma unroll
-      for (int q = 0; q < K; ++q)
+        for (int q = 0; q < K; ++q) {
+          const double v = s_ay[u + q];
+          yw[K - 1 + q] = v;
+          sy += v;
+        }
+        double xv[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = fma(xv[q], yw[q - k + K - 1], acc[k]);
+        for (int q = 0; q < K; ++q) xv[q] = s_ax[xoff + u + q];
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            acc[k] = fma(xv[q], yw[q - k + K - 1], acc[k]);
+      }
     }
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
       const double v = acc[k] - a.muX[i + d] * sy;
-      cov[k] = d < a.d_hi ? (DT)v : nan_of<DT>();
+      cov[k] = diag_valid(d) ? (DT)v : nan_of<DT>();
     }
   }
 
+  // FP64 form: seeds computed by the exact-summation seed kernel.
   __device__ __forceinline__ void seed_from(const double *src) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int64_t d = d0 + k;
-      cov[k] = d < a.d_hi ? (DT)src[k] : nan_of<DT>();
+      const double v = src[k];
+      cov[k] = diag_valid(d0 + k) ? (DT)v : nan_of<DT>();
     }
   }
 
@@ -280,41 +366,44 @@ struct Sweep {
   }
 
   // ------------------------------------------------------------ one row
+  // Row i = i_b + r of a block whose first row satisfies (i_b + D0) % K == 0,
+  // so diagonal k of every lane crosses the column in slot (r + k) % K.
   template <bool MASKED>
-  __device__ __forceinline__ void row(int r, int64_t i, Cols<DT> &cur,
-                                      Cols<DT> &nxt, DT trow) {
-    // the slot first used on the next row: decode its threshold now
-    decode_slot(nxt.s[r]);
-    const DT rdf = dfY()[i], rdg = dgY()[i], rin = inY()[i];
+  __device__ __forceinline__ void row(int r, int64_t i, const Terms<DT> *s_row) {
+    const Terms<DT> rv = s_row[r];
+    const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
     DT c[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const Slot<DT> &S = (r + k < K) ? cur.s[r + k] : nxt.s[r + k - K];
-      c[k] = cov[k] * S.in * rin;
+      c[k] = cov[k] * S[(r + k) % K].in * rin;
       if (MASKED && i + d0 + k >= a.nX) c[k] = nan_of<DT>();
     }
     if (KIND == kNN) {
-      bool hit = false;
+      // One compare per test written straight into a wave mask (keeps the
+      // compiler from merging the two tests into a NaN-canonicalising min).
+      uint64_t hit = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const Slot<DT> &S = (r + k < K) ? cur.s[r + k] : nxt.s[r + k - K];
-        if (COLS) hit |= c[k] >= S.t;
-        if (ROWS) hit |= c[k] >= trow;
+        if (COLS) hit |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].t);
+        if (ROWS) hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
       }
-      if (__builtin_amdgcn_ballot_w64(hit)) {
+      if (hit) {
         if (ROWS) nn_row_update(c, i, trow);
         if (COLS) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            Slot<DT> &S = (r + k < K) ? cur.s[r + k] : nxt.s[r + k - K];
-            if (c[k] >= S.t) {
+            Slot<DT> &s = S[(r + k) % K];
+            if (c[k] >= s.t) {
               const float f = (float)c[k];
               const uint64_t key = make_key(f, static_cast<uint32_t>(i));
-              const unsigned long long old = atomicMax(
+              atomicMax(
                   reinterpret_cast<unsigned long long *>(a.keyX + i + d0 + k),
                   static_cast<unsigned long long>(key));
-              const uint64_t best = old > key ? old : key;
-              S.t = Thresh<DT>::from_key_hi(static_cast<uint32_t>(best >> 32));
+              // the column's best is now at least f (no need to wait for the
+              // atomic's return value); the tighter threshold travels with
+              // the column to the next lanes
+              const DT tf = Thresh<DT>::from_key_hi(enc_corr(f));
+              s.t = s.t > tf ? s.t : tf;
             }
           }
         }
@@ -323,10 +412,9 @@ struct Sweep {
       double rs = 0.0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        Slot<DT> &S = (r + k < K) ? cur.s[r + k] : nxt.s[r + k - K];
         const double v = c[k] > (DT)a.thr ? (double)c[k] : 0.0;
         if (ROWS) rs += v;
-        if (COLS) S.acc += v;
+        if (COLS) S[(r + k) % K].acc += v;
       }
       if (ROWS && __builtin_amdgcn_ballot_w64(rs != 0.0)) {
         rs = wave_sum(rs);
@@ -348,98 +436,103 @@ struct Sweep {
     // advance the recurrence to row i+1
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const Slot<DT> &S = (r + k < K) ? cur.s[r + k] : nxt.s[r + k - K];
-      cov[k] = fma(S.df, rdg, cov[k]);
-      cov[k] = fma(rdf, S.dg, cov[k]);
-    }
-    // cur.s[r] held column i + d0 (diagonal 0 of the lane): it leaves this
-    // lane now.
-    if (KIND == kSum && COLS) {
-      const double ex = cur.s[r].acc;
-      const double rcv = __shfl_down(ex, 1);
-      nxt.s[r].acc = lane == kWave - 1 ? 0.0 : rcv;
-      if (lane == 0 && ex != 0.0) atomicAdd(a.sumX + i + D0, ex);
+      const Slot<DT> &s = S[(r + k) % K];
+      cov[k] = fma(s.df, rdg, cov[k]);
+      cov[k] = fma(rdf, s.dg, cov[k]);
     }
     if (KIND == kMatrix) cells_row_end();
   }
 
-  // Flushes what a lane still holds when its segment ends after row i_last
-  // of the block that started at i_b (cur/nxt as in that block).
-  __device__ __forceinline__ void segment_end(int64_t i_b, int r_last,
-                                              Cols<DT> &cur, Cols<DT> &nxt) {
+  // Slot r now holds the column that leaves this lane (column i + d0): shift
+  // every lane's slot r one lane down.  Lane 0's column leaves the band (its
+  // partial sum is flushed); lane 63 takes the entering column.
+  __device__ __forceinline__ void shift(int r, int64_t i,
+                                        const Terms<DT> *s_col) {
+    Slot<DT> &s = S[r];
+    const Terms<DT> e = s_col[r];
+    if (KIND == kSum && COLS) {
+      if (lane == 0 && s.acc != 0.0) atomicAdd(a.sumX + i + D0, s.acc);
+      s.acc = shl1(s.acc, 0.0);
+    }
+    s.df = shl1(s.df, e.df);
+    s.dg = shl1(s.dg, e.dg);
+    s.in = shl1(s.in, e.in);
+    if (KIND == kNN && COLS) s.t = shl1(s.t, e.t);
+  }
+
+  // Flushes what a lane still holds when its segment ends after row i.
+  __device__ __forceinline__ void segment_end(int r, int64_t i) {
     if (KIND == kSum && COLS) {
 #pragma unroll
-      for (int s = 0; s < K; ++s) {
-        if (s > r_last && cur.s[s].acc != 0.0)
-          atomicAdd(a.sumX + i_b + d0 + s, cur.s[s].acc);
-        if (s <= r_last && nxt.s[s].acc != 0.0)
-          atomicAdd(a.sumX + i_b + K + d0 + s, nxt.s[s].acc);
+      for (int k = 0; k < K; ++k) {
+        const Slot<DT> &s = S[(r + k) % K];
+        if (s.acc != 0.0) atomicAdd(a.sumX + i + d0 + k, s.acc);
       }
     }
     if (KIND == kMatrix) cells_flush_all();
   }
 
-  // One block of K rows starting at i_b; cur/nxt swap roles between blocks.
+  // One block of K rows starting at i_b.  `pre` holds this block's staged
+  // terms on entry and the next block's on exit.
   template <bool MASKED>
-  __device__ __forceinline__ bool block(int64_t i_b, int64_t re, Cols<DT> &cur,
-                                        Cols<DT> &nxt, DT *s_rowt,
+  __device__ __forceinline__ bool block(int64_t i_b, int64_t re, Pre &pre,
+                                        Terms<DT> *s_row, Terms<DT> *s_col,
                                         bool first) {
-    if (KIND == kNN && ROWS) {
-      if (lane < K) {
-        int64_t ri = i_b + lane;
-        ri = ri < a.nY ? ri : a.nY - 1;
-        s_rowt[lane] = Thresh<DT>::from_key_hi(
-            reinterpret_cast<const uint32_t *>(a.keyY)[2 * ri + 1]);
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
+    publish(pre, s_row, s_col);
+    prefetch(pre, i_b + K);
+    // keep the look-ahead loads where they are issued: the scheduler would
+    // otherwise sink them next to their first use a block later
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < K; ++r) {
       const int64_t i = i_b + r;
       if (MASKED && i >= re) return true;
       if (KIND == kMatrix) cells_row_start(first && r == 0);
-      const DT trow = (KIND == kNN && ROWS) ? s_rowt[r] : (DT)0;
-      row<MASKED>(r, i, cur, nxt, trow);
+      row<MASKED>(r, i, s_row);
       if (MASKED && i + 1 == re) {
-        segment_end(i_b, r, cur, nxt);
+        segment_end(r, i);
         return true;
       }
-      // refill: cur.s[r] becomes slot r of the next block's look-ahead set
-      load_slot(cur.s[r], i_b + 2 * K + d0 + r);
+      shift(r, i, s_col);
     }
-    if (KIND == kNN && ROWS) __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_wave_barrier();
     return false;
   }
 
-  // Rows [rs, re) of the band; all band diagonals are valid for rows < rfull.
+  // Rows [rs, re) of the band (rs % K == 0); all band diagonals are valid
+  // for rows < rfull.
   __device__ __forceinline__ void segment(int64_t rs, int64_t re,
-                                          int64_t rfull, DT *s_rowt) {
+                                          int64_t rfull, Terms<DT> *s_row,
+                                          Terms<DT> *s_col, double *s_ax,
+                                          double *s_ay) {
     if (rs >= re) return;
-    Cols<DT> A, B;
+    Pre pre;
+    prefetch(pre, rs);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
-      load_slot(A.s[s], rs + d0 + s);
-      load_slot(B.s[s], rs + d0 + K + s);
-    }
-#pragma unroll
-    for (int s = 0; s < K; ++s) decode_slot(A.s[s]);
+    for (int k = 0; k < K; ++k) load_slot(S[k], rs + d0 + k);
     if (KIND == kMatrix) cells_begin(rs);
     int64_t i = rs;
     const int64_t fast_end = re < rfull ? re : rfull;
     bool first = true;
-    // The last row always falls in the masked tail below, which flushes.
-    while (i + 2 * K < re && i + 2 * K <= fast_end) {
-      if (sizeof(DT) == 4 && i != rs && ((i - rs) % a.anchor) == 0) anchor(i);
-      block<false>(i, re, A, B, s_rowt, first);
-      first = false;
-      block<false>(i + K, re, B, A, s_rowt, false);
-      i += 2 * K;
-    }
-    for (;;) {
-      if (block<true>(i, re, A, B, s_rowt, first)) return;
+    // Fast region; the last row always falls in the masked tail below,
+    // which flushes.
+    int64_t to_anchor = a.anchor;
+    while (i + K < re && i + K <= fast_end) {
+      if (sizeof(DT) == 4) {
+        if (to_anchor <= 0) {
+          anchor(i, s_ax, s_ay);
+          to_anchor = a.anchor;
+        }
+        to_anchor -= K;
+      }
+      block<false>(i, re, pre, s_row, s_col, first);
       first = false;
       i += K;
-      if (block<true>(i, re, B, A, s_rowt, false)) return;
+    }
+    for (;;) {
+      if (block<true>(i, re, pre, s_row, s_col, first)) return;
+      first = false;
       i += K;
     }
   }
@@ -447,17 +540,22 @@ struct Sweep {
 
 template <typename DT, int KIND, bool COLS, bool ROWS>
 __global__ __launch_bounds__(kWave) void sweep_kernel(SweepArgs a) {
-  __shared__ DT s_rowt[K];
+  __shared__ Terms<DT> s_row[kWave];
+  __shared__ Terms<DT> s_col[kWave];
+  // FP32 form only: anchor staging (unreferenced, hence unallocated, in FP64)
+  __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
+  __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
   const int lane = threadIdx.x;
   for (;;) {
     unsigned int b = 0;
     if (lane == 0) b = atomicAdd(a.counter, 1u);
     b = (unsigned int)__builtin_amdgcn_readfirstlane((int)b);
     if (b >= a.nbands) break;
-    const int64_t D0 = a.d_lo + (int64_t)b * kBand;
+    const int64_t D0 = a.d_base + (int64_t)b * kBand;
     Sweep<DT, KIND, COLS, ROWS> sw(a, lane, D0);
+    const int64_t d_first = D0 > a.d_lo ? D0 : a.d_lo;
     const int64_t dlast = D0 + kBand < a.d_hi ? D0 + kBand - 1 : a.d_hi - 1;
-    const int64_t rows0 = a.nY < a.nX - D0 ? a.nY : a.nX - D0;
+    const int64_t rows0 = a.nY < a.nX - d_first ? a.nY : a.nX - d_first;
     const int64_t rfull0 = a.nY < a.nX - dlast ? a.nY : a.nX - dlast;
     const int64_t r0 = a.band_r0[b];
     // segment 0: rows [r0, rows0) from the band's own seed at row r0;
@@ -469,11 +567,11 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(SweepArgs a) {
       if (rs >= re) continue;
       if (sizeof(DT) == 8) {
         sw.seed_from(seg == 0 ? a.seedB + (int64_t)b * kBand + (int64_t)lane * K
-                              : a.seed0 + (sw.d0 - a.d_lo));
+                              : a.seed0 + (sw.d0 - a.d_base));
       } else {
-        sw.anchor(rs);
+        sw.anchor(rs, s_ax, s_ay);
       }
-      sw.segment(rs, re, rfull, s_rowt);
+      sw.segment(rs, re, rfull, s_row, s_col, s_ax, s_ay);
     }
   }
 }
