@@ -1,0 +1,217 @@
+"""Headline benchmark: self-join pairs/sec at n = 2^22, m = 1024 -- the plain
+FP64 path and the mixed FP32-recurrence / FP64-anchor form, with the mixed
+form's error against FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--anchor R] [--form both|fp64|mixed]
+
+One step = one complete self-join (profile reset, statistics, exact seeds,
+the diagonal sweep, profile conversion) of the workload below, with the input
+series already resident in HBM.  Prints ONE JSON line.
+
+Workload: a random walk of 4194304 float64 samples (cumulative sum of N(0,1),
+numpy PCG64 seed 2) with 0.05 % of the samples replaced by NaN in runs of
+1-50 samples placed uniformly by seed; window m = 1024; exclusion zone m/4;
+profile type 1NN_INDEX.  Pairs counted = (n_sub - e)(n_sub - e + 1) / 2 with
+n_sub = n - m + 1, e = m // 4.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-scrimp-large_amd"))
+
+N = 1 << 22
+M = 1024
+SEED = 2
+NAN_FRACTION = 0.0005
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix alike)
+FLOP_PER_PAIR = 6         # 2 FMA (covariance update) + 2 MUL (correlation)
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def make_series(n=N, seed=SEED, nan_fraction=NAN_FRACTION):
+    """cumsum N(0,1) with NaN runs of 1..50 samples until >= nan_fraction*n
+    samples are NaN."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = np.cumsum(rng.standard_normal(n))
+    target = int(round(nan_fraction * n))
+    mask = np.zeros(n, dtype=bool)
+    while mask.sum() < target:
+        L = int(rng.integers(1, 51))
+        s = int(rng.integers(0, n - L + 1))
+        mask[s:s + L] = True
+    a[mask] = np.nan
+    return a
+
+
+def pair_count(n, m):
+    ns = n - m + 1
+    e = m // 4
+    return (ns - e) * (ns - e + 1) // 2
+
+
+def cpu_baseline(a, m):
+    """SCAMP's own CPU double path (compiled from the reference's sources into
+    oracle/_ref, or the oracle's C restatement when that is absent), timed on
+    a bounded sample: the self-join of the first 2^19 samples of the same
+    series with the same window."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    n_s = 1 << 19
+    sample = a[:n_s]
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    kind = "reference" if O.ref_available() else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        O.join(sample, None, m, impl="reference", num_workers=cores)
+    else:
+        cores = 1
+        O.join(sample, None, m, impl="port", num_workers=1)
+    dt = time.perf_counter() - t0
+    pairs = pair_count(n_s, m)
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": cores, "kind": kind,
+            "sample": f"self-join of the first {n_s} samples of the benchmark series "
+                      f"(m={m}, {pairs:.3e} pairs) in {dt:.2f} s with max_tile_size 2^17"}
+
+
+def load_traffic(name):
+    """HBM bytes per sweep launch measured with rocprofv3 --pmc (committed
+    summary), or None."""
+    path = os.path.join(PROFILES, name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--anchor", type=int, default=0,
+                    help="mixed form re-anchor interval (0 = autotune)")
+    ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both",
+                    help="which forms to time (profiling runs time one form)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the join on a single "
+              "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from scamp_amd import _capi as C
+    from scamp_amd import tuning
+
+    dev = torch.device("cuda:0")
+    a = make_series()
+    ns = N - M + 1
+    pairs = pair_count(N, M)
+    a_d = torch.from_numpy(a).to(dev)
+    prof = torch.empty(ns, dtype=torch.int64, device=dev)
+    init = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
+
+    def run(params):
+        prof.fill_(init)
+        torch.cuda.synchronize()
+        C.join_device(a_d.data_ptr(), N, 0, 0, params, prof.data_ptr())
+        return C.context(0).timing()
+
+    p64 = C.make_params(M, precision="double")
+    anchor = args.anchor or (tuning.autotune_anchor(a, M) if args.form != "fp64" else 0)
+    p32 = C.make_params(M, precision="mixed", anchor_interval=anchor)
+
+    results = {}
+    forms = [f for f in (("fp64", p64), ("mixed", p32))
+             if args.form in ("both", f[0])]
+    for name, params in forms:
+        for _ in range(args.warmup):
+            run(params)
+        torch.cuda.synchronize()
+        sweeps = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sweeps.append(run(params)["sweep_ms"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        results[name] = {"ms_per_step": dt * 1e3 / args.steps,
+                         "sweep_ms": float(np.mean(sweeps)),
+                         "profile": prof.cpu().numpy().view(np.uint64).copy()}
+
+    mixed = None
+    if "mixed" in results:
+        v32 = pairs / (results["mixed"]["ms_per_step"] / 1e3)
+        mixed = {"value": v32, "unit": "pairs/s",
+                 "ms_per_step": results["mixed"]["ms_per_step"],
+                 "dtype": "f32 compensated recurrence, f64 statistics/anchors",
+                 "anchor_interval": anchor}
+    if len(results) == 2:
+        c64, i64 = C.split_mp_entry(results["fp64"]["profile"])
+        c32, i32 = C.split_mp_entry(results["mixed"]["profile"])
+        valid = c64 > -1
+        dcorr = float(np.max(np.abs(c32[valid].astype(np.float64) - c64[valid])))
+        agree = float(np.mean(i32[valid] == i64[valid]))
+        # exact FP64 correlation gap between the two matches where they differ
+        rows = np.nonzero(valid & (i32 != i64) & (c32 > -1))[0]
+        gap = 0.0
+        if rows.size:
+            gap = float(np.max(np.abs(tuning._exact_corr(a, a, M, rows, i64[rows])
+                                      - tuning._exact_corr(a, a, M, rows, i32[rows]))))
+        mixed.update({
+            "speedup_vs_fp64": results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"],
+            "max_abs_dcorr_vs_fp64": dcorr, "index_agreement_vs_fp64": agree,
+            "index_disagreements": int(rows.size),
+            "max_exact_gap_at_disagreements": gap,
+            "meets_1e-5": dcorr <= 1e-5})
+
+    head = "fp64" if "fp64" in results else "mixed"
+    v = pairs / (results[head]["ms_per_step"] / 1e3)
+    sweep_s = results[head]["sweep_ms"] / 1e3
+    achieved = pairs * FLOP_PER_PAIR / sweep_s / 1e12
+    line = {
+        "metric": "self-join pairs/sec (n=2^22,m=1024): FP64 vs mixed at <=1e-5 corr error",
+        "value": v,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": results[head]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "self-join 1NN_INDEX of a random walk, n=4194304 float64 "
+                               "(PCG64 seed 2, 0.05% NaN in runs of 1-50), m=1024, "
+                               f"exclusion m/4; value = {head} form",
+                   "n": N, "window": M, "pairs": pairs, "profile_type": "1NN_INDEX"},
+        "mixed": mixed,
+        "roofline": {"bound": "mfma", "pipe": "VALU FP64 (MI355X FP64 peak 78.6 TFLOP/s, "
+                                              "vector and matrix alike)",
+                     "kernel": f"sweep_kernel<{'double' if head == 'fp64' else 'float'},1NN>",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "flop_per_pair": FLOP_PER_PAIR,
+                     "sweep_ms_per_launch": results[head]["sweep_ms"],
+                     "traffic": load_traffic("sweep_fp64_k2_traffic.json")},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a, M)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -51,6 +51,16 @@ hipError_t launch_f32_to_enc(float *p, int64_t n, hipStream_t s);
 hipError_t launch_enc_to_f32(float *p, int64_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
 
+// ---- backward of the differentiable join (backward_kernel.hip) ------------
+// grad_t[b, :] += sum over profile entries i of grad_p[b, i] * d corr(i, I_i)
+// / d T[b, :] for `batch` series of length n stored back to back; mean/inorm
+// rows have `stat_stride` entries per series.
+hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
+                                   int64_t batch, const double *mean,
+                                   const double *inorm, int64_t stat_stride,
+                                   const int32_t *index, const double *grad_p,
+                                   double *grad_t, hipStream_t s);
+
 // ---- the diagonal sweep (sweep_kernel.hip) --------------------------------
 enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3 };
 

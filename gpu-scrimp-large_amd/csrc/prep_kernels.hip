@@ -20,12 +20,43 @@ constexpr int kStatBlock = 256;
 constexpr int kStatChunk = 1024;
 constexpr double kFlatness = 1e-13;
 
+// The cleaned series replaces each non-finite sample by the nearest finite
+// sample before it (after it, for a leading run; 0 beyond kHoldSearch).
+// SCAMP writes 0 there (src/cpu_stats.cpp:17-23).  Any value gives the same
+// correlations -- every window that contains the sample is masked NaN, and
+// the statistics, seeds and recurrence all read this same cleaned series, so
+// the covariance of every valid window pair is the exact one -- but holding
+// the level avoids update terms as large as the series itself around NaN
+// gaps, which the FP32 recurrence could not absorb without losing digits.
+constexpr int kHoldSearch = 4096;
+
 __global__ void clean_kernel(const double *__restrict__ T, int64_t n,
                              double *__restrict__ clean, int64_t clean_len) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= clean_len) return;
-  double v = i < n ? T[i] : 0.0;
-  clean[i] = isfinite(v) ? v : 0.0;
+  if (i >= n) {
+    clean[i] = 0.0;
+    return;
+  }
+  double v = T[i];
+  if (!isfinite(v)) {
+    v = 0.0;
+    bool found = false;
+    for (int64_t j = i - 1; j >= 0 && j >= i - kHoldSearch; --j) {
+      if (isfinite(T[j])) {
+        v = T[j];
+        found = true;
+        break;
+      }
+    }
+    for (int64_t j = i + 1; !found && j < n && j <= i + kHoldSearch; ++j) {
+      if (isfinite(T[j])) {
+        v = T[j];
+        found = true;
+      }
+    }
+  }
+  clean[i] = v;
 }
 
 // One thread per window.  The block's samples are staged through LDS in
@@ -33,10 +64,11 @@ __global__ void clean_kernel(const double *__restrict__ T, int64_t n,
 // with an error-free two-sum accumulation and flags non-finite samples, pass
 // 2 accumulates the squared deviations from the mean.
 __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
-    const double *__restrict__ T, int64_t n, int64_t m,
-    double *__restrict__ means, double *__restrict__ inorms, int64_t ns,
-    int64_t stat_len) {
+    const double *__restrict__ T, const double *__restrict__ clean, int64_t n,
+    int64_t m, double *__restrict__ means, double *__restrict__ inorms,
+    int64_t ns, int64_t stat_len) {
   __shared__ double tile[kStatBlock + kStatChunk];
+  __shared__ bool flag[kStatBlock + kStatChunk];
   const int64_t w0 = blockIdx.x * (int64_t)kStatBlock;
   const int64_t w = w0 + threadIdx.x;
   double hi = 0, lo = 0;
@@ -46,17 +78,18 @@ __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
     __syncthreads();
     for (int64_t j = threadIdx.x; j < span; j += kStatBlock) {
       int64_t g = w0 + t0 + j;
-      tile[j] = g < n ? T[g] : 0.0;
+      // stage the cleaned sample and whether the raw one was non-finite
+      const double raw = g < n ? T[g] : 0.0;
+      tile[j] = raw;
+      if (!isfinite(raw)) tile[j] = clean[g];
+      flag[j] = g < n && !isfinite(raw);
     }
     __syncthreads();
     const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
     if (w < ns) {
       for (int64_t t = 0; t < tend; ++t) {
-        double v = tile[threadIdx.x + t];
-        if (!isfinite(v)) {
-          bad = true;
-          v = 0.0;
-        }
+        const double v = tile[threadIdx.x + t];
+        bad |= flag[threadIdx.x + t];
         double s = hi + v;
         double z = s - hi;
         lo += (hi - (s - z)) + (v - z);
@@ -71,8 +104,7 @@ __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
     __syncthreads();
     for (int64_t j = threadIdx.x; j < span; j += kStatBlock) {
       int64_t g = w0 + t0 + j;
-      double v = g < n ? T[g] : 0.0;
-      tile[j] = isfinite(v) ? v : 0.0;
+      tile[j] = g < n ? clean[g] : 0.0;
     }
     __syncthreads();
     const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
@@ -201,7 +233,7 @@ hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
                      0, s, T, n, clean, clean_len);
   hipLaunchKernelGGL(window_stats_kernel,
                      dim3(blocks_for(stat_len, kStatBlock)), dim3(kStatBlock),
-                     0, s, T, n, m, means, inorms, ns, stat_len);
+                     0, s, T, clean, n, m, means, inorms, ns, stat_len);
   hipLaunchKernelGGL(dfdg_kernel, dim3(blocks_for(stat_len, 256)), dim3(256),
                      0, s, clean, m, means, ns, df, dg, stat_len);
   return hipGetLastError();

@@ -1,0 +1,249 @@
+// SCAMP's C++ API (include/scamp.hpp) over the engine's C ABI.
+//
+// do_SCAMP keeps the reference's contract (src/SCAMP.cpp:261-334): validate
+// the arguments, run the join, leave the profile(s) in args->profile_a / _b
+// (which the caller pre-allocated, e.g. with InitProfileMemory, and which are
+// read as the prior best-so-far), throw SCAMPException on failure.
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "scamp.hpp"
+#include "scamp_hip.h"
+
+namespace SCAMP {
+
+void Profile::Alloc(size_t size, int64_t matrix_height, int64_t matrix_width,
+                    float default_thresh) {
+  switch (type) {
+    case PROFILE_TYPE_SUM_THRESH:
+      data.emplace_back();
+      data[0].double_value.resize(size, 0);
+      break;
+    case PROFILE_TYPE_1NN:
+      data.emplace_back();
+      data[0].float_value.resize(size, -2.0);
+      break;
+    case PROFILE_TYPE_1NN_INDEX: {
+      mp_entry e;
+      e.ints[1] = -1u;
+      e.floats[0] = -2.0;
+      data.emplace_back();
+      data[0].uint64_value.resize(size, e.ulong);
+      break;
+    }
+    case PROFILE_TYPE_FREQUENCY_THRESH:
+      data.emplace_back();
+      data[0].uint64_value.resize(size, 0);
+      break;
+    case PROFILE_TYPE_APPROX_ALL_NEIGHBORS:
+      data.emplace_back();
+      thresholds.resize(size, default_thresh);
+      break;
+    case PROFILE_TYPE_MATRIX_SUMMARY:
+      data.emplace_back();
+      data[0].float_value.resize(matrix_height * matrix_width, -2.0);
+      break;
+    default:
+      break;
+  }
+}
+
+// src/common.cpp:417-436 (same checks, same messages)
+void SCAMPArgs::validate() {
+  if (window < 3) {
+    throw SCAMPException("Error: Subsequence length must be at least 3");
+  }
+  if (max_tile_size < 1024) {
+    throw SCAMPException("Error: max tile size must be at least 1024");
+  }
+  if (max_tile_size / 2 < window) {
+    throw SCAMPException(
+        "Error: Tile length and width must be at least 2x larger than the "
+        "window size");
+  }
+  if (timeseries_a.size() < window || (has_b && timeseries_b.size() < window)) {
+    throw SCAMPException(
+        "Error: Input time series must be at least 'subesequence window "
+        "size' in length");
+  }
+}
+
+void SCAMPArgs::print() {
+  std::cout << "window: " << window << std::endl;
+  std::cout << "max_tile_size: " << max_tile_size << std::endl;
+  std::cout << "has_b: " << has_b << std::endl;
+  std::cout << "keep_rows_separate: " << keep_rows_separate << std::endl;
+  std::cout << "computing_rows: " << computing_rows << std::endl;
+  std::cout << "computing_columns: " << computing_columns << std::endl;
+  std::cout << "is_aligned: " << is_aligned << std::endl;
+  std::cout << "profile_type: " << profile_type << std::endl;
+  std::cout << "precision_type: " << precision_type << std::endl;
+  std::cout << "distance_threshold: " << distance_threshold << std::endl;
+}
+
+std::string getSCAMPErrorString(SCAMPError_t err) {
+  switch (err) {
+    case SCAMP_NO_ERROR: return "SCAMP_NO_ERROR";
+    case SCAMP_FUNCTIONALITY_UNIMPLEMENTED:
+      return "SCAMP_FUNCTIONALITY_UNIMPLEMENTED";
+    case SCAMP_TILE_ILLEGAL_TYPE: return "SCAMP_TILE_ILLEGAL_TYPE";
+    case SCAMP_CUDA_ERROR: return "SCAMP_CUDA_ERROR";
+    case SCAMP_CUFFT_ERROR: return "SCAMP_CUFFT_ERROR";
+    case SCAMP_CUFFT_EXEC_ERROR: return "SCAMP_CUFFT_EXEC_ERROR";
+    case SCAMP_DIM_INCOMPATIBLE: return "SCAMP_DIM_INCOMPATIBLE";
+  }
+  return "SCAMP_UNKNOWN_ERROR";
+}
+
+namespace {
+
+// One context per device, created on first use and kept for the process.
+scamp_hip_ctx *context_for(int device) {
+  static std::mutex mu;
+  static std::vector<scamp_hip_ctx *> ctxs;
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0) throw SCAMPException("Error: invalid device id");
+  if ((size_t)device >= ctxs.size()) ctxs.resize(device + 1, nullptr);
+  if (!ctxs[device]) {
+    scamp_hip_ctx *c = nullptr;
+    int rc = scamp_hip_ctx_create(device, &c);
+    if (rc != SCAMP_HIP_OK) {
+      char buf[512];
+      scamp_hip_last_error(buf, sizeof(buf));
+      throw SCAMPException(std::string("Error: no HIP device usable: ") + buf);
+    }
+    ctxs[device] = c;
+  }
+  return ctxs[device];
+}
+
+void *profile_buffer(Profile &p, SCAMPProfileType t) {
+  if (p.data.empty()) return nullptr;
+  switch (t) {
+    case PROFILE_TYPE_1NN_INDEX: return p.data[0].uint64_value.data();
+    case PROFILE_TYPE_SUM_THRESH: return p.data[0].double_value.data();
+    case PROFILE_TYPE_1NN:
+    case PROFILE_TYPE_MATRIX_SUMMARY: return p.data[0].float_value.data();
+    default: return nullptr;
+  }
+}
+
+size_t profile_length(const Profile &p, SCAMPProfileType t) {
+  if (p.data.empty()) return 0;
+  switch (t) {
+    case PROFILE_TYPE_1NN_INDEX: return p.data[0].uint64_value.size();
+    case PROFILE_TYPE_SUM_THRESH: return p.data[0].double_value.size();
+    default: return p.data[0].float_value.size();
+  }
+}
+
+}  // namespace
+
+void do_SCAMP(SCAMPArgs *args, const std::vector<int> &devices,
+              int num_threads) {
+  (void)num_threads;
+  if (args == nullptr) {
+    throw SCAMPException("Error: Invalid arguments provided to SCAMP");
+  }
+  args->validate();
+  int ndev = 0;
+  scamp_hip_device_count(&ndev);
+  if (ndev == 0) {
+    throw SCAMPException(
+        "Error: no compute_resources provided (no HIP device is visible; "
+        "this engine has no CPU path)");
+  }
+  const SCAMPProfileType t = args->profile_type;
+  if (t != PROFILE_TYPE_1NN_INDEX && t != PROFILE_TYPE_1NN &&
+      t != PROFILE_TYPE_SUM_THRESH && t != PROFILE_TYPE_MATRIX_SUMMARY) {
+    throw SCAMPException("ERROR " +
+                         getSCAMPErrorString(SCAMP_FUNCTIONALITY_UNIMPLEMENTED) +
+                         " executing tile");
+  }
+  const int device = devices.empty() ? 0 : devices[0];
+  scamp_hip_ctx *ctx = context_for(device);
+
+  const size_t na = args->timeseries_a.size();
+  const size_t nb = args->has_b ? args->timeseries_b.size() : na;
+  const size_t m = args->window;
+  // the caller pre-allocates the profiles (InitProfileMemory)
+  const bool matrix = t == PROFILE_TYPE_MATRIX_SUMMARY;
+  const size_t need_a =
+      matrix ? (size_t)(args->matrix_height * args->matrix_width) : na - m + 1;
+  if (args->profile_a.type != t || profile_length(args->profile_a, t) < need_a) {
+    throw SCAMPException("Error: profile_a is not allocated for this join");
+  }
+  const bool sep = !matrix && args->computing_rows && args->keep_rows_separate;
+  if (sep && (args->profile_b.type != t ||
+              profile_length(args->profile_b, t) < nb - m + 1)) {
+    throw SCAMPException("Error: profile_b is not allocated for this join");
+  }
+
+  scamp_hip_params p{};
+  p.window = (int64_t)m;
+  p.profile_type = t;
+  p.precision = args->precision_type;
+  p.has_b = args->has_b;
+  p.computing_rows = args->computing_rows;
+  p.computing_columns = args->computing_columns;
+  p.keep_rows_separate = args->keep_rows_separate;
+  p.is_aligned = args->is_aligned;
+  p.silent_mode = args->silent_mode;
+  p.distributed_start_row = args->distributed_start_row;
+  p.distributed_start_col = args->distributed_start_col;
+  p.distance_threshold = args->distance_threshold;
+  p.matrix_height = args->matrix_height;
+  p.matrix_width = args->matrix_width;
+  p.max_tile_size = (int64_t)args->max_tile_size;
+  p.anchor_interval = args->anchor_interval;
+
+  int rc = scamp_hip_join(ctx, args->timeseries_a.data(), na,
+                          args->has_b ? args->timeseries_b.data() : nullptr,
+                          args->has_b ? nb : 0, &p,
+                          profile_buffer(args->profile_a, t),
+                          sep ? profile_buffer(args->profile_b, t) : nullptr);
+  if (rc != SCAMP_HIP_OK) {
+    char buf[512];
+    scamp_hip_last_error(buf, sizeof(buf));
+    throw SCAMPException(std::string(buf));
+  }
+}
+
+void do_SCAMP(SCAMPArgs *args) { do_SCAMP(args, std::vector<int>(), 0); }
+
+}  // namespace SCAMP
+
+// src/scamp_utils.cpp:226-245
+bool InitProfileMemory(SCAMP::SCAMPArgs *args) {
+  int64_t profile_a_size = (int64_t)args->timeseries_a.size() - (int64_t)args->window + 1;
+  int64_t profile_b_size =
+      args->has_b ? (int64_t)args->timeseries_b.size() - (int64_t)args->window + 1
+                  : profile_a_size;
+  if (profile_a_size <= 0 || (args->keep_rows_separate && profile_b_size <= 0)) {
+    return false;
+  }
+  args->profile_a.data.clear();
+  args->profile_a.Alloc(profile_a_size, args->matrix_height, args->matrix_width,
+                        args->distance_threshold);
+  if (args->keep_rows_separate) {
+    args->profile_b.data.clear();
+    args->profile_b.Alloc(profile_b_size, args->matrix_height,
+                          args->matrix_width, args->distance_threshold);
+  }
+  return true;
+}
+
+// src/scamp_utils.cpp:127-141
+double ConvertToEuclidean(double val, int window) {
+  if (val < -1) return NAN;
+  return std::sqrt(std::max(2.0 * window * (1.0 - val), 0.0));
+}
+
+double CleanupPearson(double val) {
+  if (val < -1) return NAN;
+  return val;
+}

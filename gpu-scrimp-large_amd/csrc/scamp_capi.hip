@@ -257,7 +257,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   if (pt == SCAMP_HIP_MATRIX_SUMMARY &&
       (p->matrix_height <= 0 || p->matrix_width <= 0))
     return fail(SCAMP_HIP_INVALID_ARGS, "matrix height and width must be > 0");
-  int64_t anchor = p->anchor_interval > 0 ? p->anchor_interval : 2048;
+  int64_t anchor = p->anchor_interval > 0 ? p->anchor_interval : 8192;
   anchor = std::max<int64_t>(16, (anchor + 15) / 16 * 16);
 
   hipStream_t s = ctx->stream;
@@ -600,6 +600,43 @@ int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
                              hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
   } catch (const HipFailure &f) {
+    return fail(SCAMP_HIP_DEVICE_ERROR,
+                std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
+                               int64_t window, int64_t batch,
+                               const int32_t *d_index,
+                               const double *d_grad_corr, double *d_grad_T) {
+  if (!ctx || !d_T || !d_index || !d_grad_corr || !d_grad_T)
+    return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  const int64_t m = window;
+  if (m < 3 || (int64_t)n < m || batch < 1)
+    return fail(SCAMP_HIP_INVALID_ARGS, "invalid window, length or batch");
+  try {
+    HIP_CHECK(hipSetDevice(ctx->device));
+    DevArena ar;
+    hipStream_t s = ctx->stream;
+    const int64_t ns = (int64_t)n - m + 1;
+    const int64_t stat_len = ns + kColumnPad;
+    double *mean = ar.alloc<double>(batch * stat_len);
+    double *inorm = ar.alloc<double>(batch * stat_len);
+    double *clean = ar.alloc<double>((int64_t)n + kColumnPad);
+    double *df = ar.alloc<double>(stat_len);
+    double *dg = ar.alloc<double>(stat_len);
+    for (int64_t b = 0; b < batch; ++b)
+      HIP_CHECK(launch_stats(d_T + b * n, (int64_t)n, m, clean,
+                             (int64_t)n + kColumnPad, mean + b * stat_len,
+                             inorm + b * stat_len, df, dg, stat_len, s));
+    HIP_CHECK(hipMemsetAsync(d_grad_T, 0, sizeof(double) * n * batch, s));
+    HIP_CHECK(launch_pearson_backward(d_T, (int64_t)n, m, batch, mean, inorm,
+                                      stat_len, d_index, d_grad_corr, d_grad_T,
+                                      s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (const HipFailure &f) {
+    (void)hipStreamSynchronize(ctx->stream);
     return fail(SCAMP_HIP_DEVICE_ERROR,
                 std::string(f.what) + ": " + hipGetErrorString(f.err));
   }

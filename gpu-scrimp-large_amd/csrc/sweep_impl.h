@@ -41,11 +41,13 @@
 //  * MATRIX_SUMMARY: exact max-pooling; each lane keeps two running cell
 //    maxima (its 8 columns straddle at most one cell boundary) and flushes a
 //    cell with a 32-bit atomicMax when its columns or the row leave it.
+#pragma once
+
 #include "device_common.h"
 #include "kernels.h"
 
 namespace scamp_amd {
-namespace {
+namespace sweep {
 
 constexpr int K = kDiagsPerLane;
 constexpr int kAnchorChunk = 256;  // u-samples per LDS stage of an anchor
@@ -103,7 +105,14 @@ struct Sweep {
   const int lane;
   const int64_t D0;      // first diagonal of the band (multiple of K)
   const int64_t d0;      // first diagonal of this lane
+  // FP64 form: cov[k] is the covariance of diagonal d0+k at the current row.
+  // FP32 form: the covariance is hi[k] + cov[k]; hi[k] is its leading part
+  // and cov[k] collects the row increments of the current block on top of
+  // the low part, so FP32 rounding hits the small partial sum every row and
+  // the large leading part only once per block (compensated, by Fast2Sum).
+  static constexpr bool kComp = sizeof(DT) == 4;
   DT cov[K];
+  DT hi[K];
   Slot<DT> S[K];         // column slots, indexed by column mod K
   // MATRIX_SUMMARY state
   CellAcc dir, tra;
@@ -235,7 +244,21 @@ struct Sweep {
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
       const double v = acc[k] - a.muX[i + d] * sy;
-      cov[k] = diag_valid(d) ? (DT)v : nan_of<DT>();
+      const DT h = (DT)v;
+      hi[k] = diag_valid(d) ? h : nan_of<DT>();
+      cov[k] = (DT)(v - (double)h);
+    }
+  }
+
+  // FP32 form, end of a block: folds the block's partial sum into the
+  // leading part; the rounding error of that addition (exact by Fast2Sum
+  // while |hi| >= |partial|) seeds the next block's partial sum.
+  __device__ __forceinline__ void renormalise() {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const DT s = hi[k] + cov[k];
+      cov[k] = cov[k] - (s - hi[k]);
+      hi[k] = s;
     }
   }
 
@@ -375,7 +398,8 @@ struct Sweep {
     DT c[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      c[k] = cov[k] * S[(r + k) % K].in * rin;
+      const DT cv = kComp ? hi[k] + cov[k] : cov[k];
+      c[k] = cv * S[(r + k) % K].in * rin;
       if (MASKED && i + d0 + k >= a.nX) c[k] = nan_of<DT>();
     }
     if (KIND == kNN) {
@@ -495,6 +519,7 @@ struct Sweep {
       }
       shift(r, i, s_col);
     }
+    if (kComp) renormalise();
     __builtin_amdgcn_wave_barrier();
     return false;
   }
@@ -592,25 +617,5 @@ hipError_t launch_kind(const SweepArgs &a, bool cols, bool rows, int grid,
   return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_sweep(const SweepArgs &a, bool fp32, SweepKind kind,
-                        bool cols, bool rows, int grid, hipStream_t s) {
-  if (!cols && !rows) return hipSuccess;
-  if (fp32) {
-    switch (kind) {
-      case kNN: return launch_kind<float, kNN>(a, cols, rows, grid, s);
-      case kSum: return launch_kind<float, kSum>(a, cols, rows, grid, s);
-      case kMatrix: return launch_kind<float, kMatrix>(a, cols, rows, grid, s);
-      default: return launch_kind<float, kMatrixSmall>(a, cols, rows, grid, s);
-    }
-  }
-  switch (kind) {
-    case kNN: return launch_kind<double, kNN>(a, cols, rows, grid, s);
-    case kSum: return launch_kind<double, kSum>(a, cols, rows, grid, s);
-    case kMatrix: return launch_kind<double, kMatrix>(a, cols, rows, grid, s);
-    default: return launch_kind<double, kMatrixSmall>(a, cols, rows, grid, s);
-  }
-}
-
+}  // namespace sweep
 }  // namespace scamp_amd

@@ -223,3 +223,13 @@ def qt_seed(X, Y, m, row, diag0, count, device=0):
     check(ctx._lib.scamp_hip_qt_seed(ctx.handle, _ptr(X), X.size, _ptr(Y), Y.size, m, row,
                                      diag0, count, _ptr(out)))
     return out
+
+
+def join_device(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=None, device=0):
+    """Device-buffer join (scamp_hip_join_device) on raw device pointers
+    (e.g. torch tensors' data_ptr()).  Blocks until the profiles are written."""
+    ctx = context(device)
+    check(ctx._lib.scamp_hip_join_device(
+        ctx.handle, ctypes.c_void_p(ptr_a), na, ctypes.c_void_p(ptr_b) if ptr_b else None,
+        nb, ctypes.byref(params), ctypes.c_void_p(ptr_pa),
+        ctypes.c_void_p(ptr_pb) if ptr_pb else None))

@@ -1,0 +1,82 @@
+"""Differentiable batched self-join (torch).
+
+``MatrixProfile.apply(T, m)`` takes a batch of series ``T`` (torch tensor
+[B, n] on the GPU, float32 or float64) and returns the 1NN_INDEX matrix
+profile as Pearson correlations ``P`` [B, n-m+1] (same dtype as T; NaN where
+a window has no match) and the indices ``I`` [B, n-m+1] (int32, -1 = none).
+
+Forward: the plain FP64 join of each series (statistics, exact seeds,
+diagonal sweep) through the C ABI's device entry point.  Backward: the HIP
+kernel ``scamp_hip_pearson_backward``, which reuses the forward's indices
+(no join is recomputed) and scatters dP_i/dT into windows i and I_i:
+    dr/dx = (y^ - r x^) / |x~|,   dr/dy = (x^ - r y^) / |y~|.
+The indices are piecewise constant in T and carry no gradient.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi as C
+
+_INIT = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
+_LOW = 0xFFFFFFFF
+
+
+def _forward_one(t64: torch.Tensor, m: int, device: int):
+    n = t64.numel()
+    prof = torch.full((n - m + 1,), _INIT, dtype=torch.int64, device=t64.device)
+    torch.cuda.synchronize(t64.device)
+    C.join_device(t64.data_ptr(), n, 0, 0, C.make_params(m, precision="double"),
+                  prof.data_ptr(), device=device)
+    corr = (prof & _LOW).to(torch.int32).view(torch.float32)
+    idx = (prof >> 32).to(torch.int32)
+    return corr, idx
+
+
+class MatrixProfile(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, T: torch.Tensor, m: int):
+        if not T.is_cuda:
+            raise ValueError("MatrixProfile needs a GPU tensor")
+        ctx.in_shape = T.shape
+        if T.dim() == 1:
+            T = T.unsqueeze(0)
+        device = T.device.index or 0
+        t64 = T.detach().to(torch.float64).contiguous()
+        B, n = t64.shape
+        P = torch.empty((B, n - m + 1), dtype=torch.float32, device=T.device)
+        I = torch.empty((B, n - m + 1), dtype=torch.int32, device=T.device)
+        for b in range(B):
+            P[b], I[b] = _forward_one(t64[b], m, device)
+        I = torch.where(P < -1, torch.full_like(I, -1), I)
+        P = torch.where(P < -1, torch.full_like(P, float("nan")), P)
+        ctx.save_for_backward(t64, I)
+        ctx.m = m
+        ctx.device = device
+        ctx.in_dtype = T.dtype
+        ctx.mark_non_differentiable(I)
+        return P.to(T.dtype), I
+
+    @staticmethod
+    def backward(ctx, grad_p, grad_i=None):
+        t64, I = ctx.saved_tensors
+        B, n = t64.shape
+        g = torch.nan_to_num(grad_p.to(torch.float64), nan=0.0).contiguous()
+        grad_t = torch.empty_like(t64)
+        torch.cuda.synchronize(t64.device)
+        lib = C.load()
+        cx = C.context(ctx.device)
+        C.check(lib.scamp_hip_pearson_backward(
+            cx.handle, ctypes.c_void_p(t64.data_ptr()), n, ctx.m, B,
+            ctypes.c_void_p(I.contiguous().data_ptr()), ctypes.c_void_p(g.data_ptr()),
+            ctypes.c_void_p(grad_t.data_ptr())))
+        return grad_t.to(ctx.in_dtype).reshape(ctx.in_shape), None
+
+
+def matrix_profile(T: torch.Tensor, m: int):
+    """(P, I) of each row of T ([B, n], or [n] -> [1, n-m+1] outputs);
+    differentiable in T through P."""
+    return MatrixProfile.apply(T, m)

@@ -79,7 +79,7 @@ typedef struct {
   int64_t matrix_width;
   int64_t max_tile_size;       /* validated as SCAMP does; the sweep is untiled */
   int64_t anchor_interval;     /* MIXED only: rows between FP64 re-anchors,
-                                  0 = library default */
+                                  0 = library default (8192) */
 } scamp_hip_params;
 
 /* Replaces: the per-worker device state of SCAMP's Tile/ExecInfo
@@ -123,10 +123,22 @@ int scamp_hip_stats(scamp_hip_ctx *ctx, const double *T, size_t n,
  * src/qt_helper.cpp:75-121, and its cuFFT kernels, src/qt_kernels.cu:3-61):
  *   qt[j] = sum_{t<m} (X[row+diag0+j+t] - mean(X window)) *
  *                     (Y[row+t] - mean(Y window)),  j < count
- * on host buffers X[nx], Y[ny] (NaN/inf read as 0, like the join). */
+ * on host buffers X[nx], Y[ny], read through the same cleaning as the join
+ * (a non-finite sample takes the value of the nearest finite one). */
 int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
                       const double *Y, size_t ny, int64_t window, int64_t row,
                       int64_t diag0, int64_t count, double *qt);
+
+/* Backward of the differentiable join (no counterpart in the reference):
+ * for `batch` series of length n stored back to back in d_T (DEVICE, float64)
+ * and the forward's 1NN indices d_index[batch][n-m+1] (int32, -1 = none),
+ *   d_grad_T[b][t] = sum_i d_grad_corr[b][i] * d corr(T[i:i+m], T[I_i:I_i+m])
+ *                                             / d T[t]
+ * with the index held fixed.  d_grad_T is overwritten.  Blocks until done. */
+int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
+                               int64_t window, int64_t batch,
+                               const int32_t *d_index,
+                               const double *d_grad_corr, double *d_grad_T);
 
 /* Timing of the last join on this context, from HIP events on its stream:
  * total, statistics+seeds, and the diagonal sweep, in milliseconds. */

@@ -141,6 +141,8 @@ def join(a, b, m, profile_type="1NN_INDEX", max_tile_size=1 << 17,
         computing_rows = not has_b
     na = a.size
     nb = b.size if has_b else na
+    if m < 1 or na < m or nb < m:
+        raise RuntimeError("oracle join failed with code 7 (series shorter than the window)")
     ca, ia = _alloc(profile_type, na - m + 1)
     cb, ib = _alloc(profile_type, nb - m + 1) if keep_rows_separate else (None, None)
     pt = PTYPES[profile_type]
