@@ -4,6 +4,7 @@ form's error against FP64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--anchor R] [--form both|fp64|mixed]
+    python bench.py --calibrate     # copy kernel of known size, for --pmc scaling
 
 One step = one complete self-join (profile reset, statistics, exact seeds,
 the diagonal sweep, profile conversion) of the workload below, with the input
@@ -96,6 +97,20 @@ def load_traffic(name):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def calibrate(steps=3, nbytes=1 << 31):
+    """A plain device copy of ``nbytes`` (elementwise kernel: read nbytes,
+    write nbytes), run ``steps`` times, so that FETCH_SIZE / WRITE_SIZE
+    counter readings can be scaled to bytes."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(steps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.mul(x, 1.0, out=y)", "bytes_read": nbytes,
+                      "bytes_written": nbytes, "launches": steps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,11 +121,16 @@ def main():
                     help="mixed form re-anchor interval (0 = autotune)")
     ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both",
                     help="which forms to time (profiling runs time one form)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only the copy kernel used to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the join on a single "
               "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate:
+        calibrate()
+        return
 
     import torch
     from scamp_amd import _capi as C
@@ -150,6 +170,14 @@ def main():
         results[name] = {"ms_per_step": dt * 1e3 / args.steps,
                          "sweep_ms": float(np.mean(sweeps)),
                          "profile": prof.cpu().numpy().view(np.uint64).copy()}
+
+    host_ms = None
+    if "fp64" in results:
+        # PCIe-inclusive reference point (host series in, host profile out);
+        # outside the timed region, never reported as `value`
+        t0 = time.perf_counter()
+        C.join(a, None, p64)
+        host_ms = (time.perf_counter() - t0) * 1e3
 
     mixed = None
     if "mixed" in results:
@@ -199,6 +227,7 @@ def main():
                                f"exclusion m/4; value = {head} form",
                    "n": N, "window": M, "pairs": pairs, "profile_type": "1NN_INDEX"},
         "mixed": mixed,
+        "host_buffers_ms_per_step": host_ms,
         "roofline": {"bound": "mfma", "pipe": "VALU FP64 (MI355X FP64 peak 78.6 TFLOP/s, "
                                               "vector and matrix alike)",
                      "kernel": f"sweep_kernel<{'double' if head == 'fp64' else 'float'},1NN>",
@@ -206,7 +235,7 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "flop_per_pair": FLOP_PER_PAIR,
                      "sweep_ms_per_launch": results[head]["sweep_ms"],
-                     "traffic": load_traffic("sweep_fp64_k2_traffic.json")},
+                     "traffic": load_traffic("sweep_fp64_selfjoin_n4194304_m1024_traffic.json")},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a, M)

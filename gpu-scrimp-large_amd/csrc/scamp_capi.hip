@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,11 +25,14 @@
 
 using namespace scamp_amd;
 
+// Calls on one context run one after another (each holds `mu`); separate
+// contexts, on the same device or not, run independently.
 struct scamp_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
   float total_ms = 0, prep_ms = 0, sweep_ms = 0;
+  std::mutex mu;
 };
 
 namespace {
@@ -479,6 +483,7 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
       p->profile_type != SCAMP_HIP_MATRIX_SUMMARY)
     return fail(SCAMP_HIP_INVALID_ARGS, "missing profile b");
   try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
     return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, p, d_profile_a,
                        d_profile_b);
@@ -507,6 +512,7 @@ int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
   const size_t len_b = sep ? (p->has_b ? nb : na) - m + 1 : 0;
   if (sep && !profile_b) return fail(SCAMP_HIP_INVALID_ARGS, "missing profile b");
   try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
     DevArena ar;
     hipStream_t s = ctx->stream;
@@ -547,6 +553,7 @@ int scamp_hip_stats(scamp_hip_ctx *ctx, const double *T, size_t n,
   if (window < 1 || (int64_t)n < window)
     return fail(SCAMP_HIP_INVALID_ARGS, "series shorter than the window");
   try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
     DevArena ar;
     hipStream_t s = ctx->stream;
@@ -577,6 +584,7 @@ int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
     return fail(SCAMP_HIP_INVALID_ARGS, "seed window outside the series");
   if (count == 0) return SCAMP_HIP_OK;
   try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
     DevArena ar;
     hipStream_t s = ctx->stream;
@@ -616,6 +624,7 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
   if (m < 3 || (int64_t)n < m || batch < 1)
     return fail(SCAMP_HIP_INVALID_ARGS, "invalid window, length or batch");
   try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
     DevArena ar;
     hipStream_t s = ctx->stream;
@@ -646,6 +655,7 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
 int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
                           double *prep_ms, double *sweep_ms) {
   if (!ctx) return fail(SCAMP_HIP_INVALID_ARGS, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
   if (total_ms) *total_ms = ctx->total_ms;
   if (prep_ms) *prep_ms = ctx->prep_ms;
   if (sweep_ms) *sweep_ms = ctx->sweep_ms;

@@ -84,7 +84,9 @@ typedef struct {
 
 /* Replaces: the per-worker device state of SCAMP's Tile/ExecInfo
  * (src/tile.cpp:141-283, src/common.cpp:109-138).  Binds a HIP device and
- * creates one stream on it. */
+ * creates one stream on it.  Calls on one context run one after another
+ * (the context holds a lock during each call); separate contexts are
+ * independent. */
 int scamp_hip_ctx_create(int device, scamp_hip_ctx **out);
 int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx);
 
