@@ -5,6 +5,7 @@ form's error against FP64.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--anchor R] [--form both|fp64|mixed]
     python bench.py --calibrate     # copy kernel of known size, for --pmc scaling
+    python bench.py --workload selfjoin-n33554432-m4096   # the large self-join
 
 One step = one complete self-join (profile reset, statistics, exact seeds,
 the diagonal sweep, profile conversion) of the workload below, with the input
@@ -53,6 +54,29 @@ def make_series(n=N, seed=SEED, nan_fraction=NAN_FRACTION):
     return a
 
 
+def make_abjoin_series(na=1 << 21, nb=1 << 23, motif=256, copies=200):
+    """AB-join inputs: A = cumsum N(0,1) (PCG64 seed 3); B = three sinusoids with
+    periods U(50, 5000) and phases U(0, 2 pi) plus N(0, 0.3) noise (PCG64 seed
+    4), with `copies` copies of one `motif`-sample random walk written over B
+    at seeded, non-overlapping offsets (one per 2*motif slot).  Returns
+    (A, B, offsets)."""
+    a = np.cumsum(np.random.Generator(np.random.PCG64(3)).standard_normal(na))
+    rng = np.random.Generator(np.random.PCG64(4))
+    t = np.arange(nb, dtype=np.float64)
+    b = np.zeros(nb)
+    for _ in range(3):
+        period = rng.uniform(50, 5000)
+        phase = rng.uniform(0, 2 * np.pi)
+        b += np.sin(2 * np.pi * t / period + phase)
+    b += rng.normal(0.0, 0.3, nb)
+    shape = np.cumsum(rng.standard_normal(motif))
+    slots = np.sort(rng.choice(nb // (2 * motif) - 1, copies, replace=False))
+    offsets = slots * (2 * motif) + rng.integers(0, motif, copies)
+    for o in offsets:
+        b[o:o + motif] = shape
+    return a, b, offsets
+
+
 def pair_count(n, m):
     ns = n - m + 1
     e = m // 4
@@ -97,6 +121,88 @@ def load_traffic(name):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def large_selfjoin(args):
+    """Self-join of a float32 random walk, n = 2^25 (PCG64 seed 5; statistics
+    in FP64 from the float32 values), m = 4096: the mixed form's 1NN_INDEX
+    profile and a 1024 x 1024 MATRIX_SUMMARY, each timed once, and the FP64
+    1NN_INDEX profile as the error reference.  Checks: every row of the
+    summary peaks at the maximum of the 1NN profile over that row band (same
+    FP32 arithmetic, so exactly); the FP64 profile at sampled rows equals
+    the exact FP64 correlation of the reported pair."""
+    import torch
+    from scamp_amd import _capi as C
+    from scamp_amd import tuning
+    n, m, H = 1 << 25, 4096, 1024
+    a = np.cumsum(np.random.Generator(np.random.PCG64(5)).standard_normal(n)
+                  .astype(np.float32), dtype=np.float32).astype(np.float64)
+    ns = n - m + 1
+    pairs = pair_count(n, m)
+    dev = torch.device("cuda:0")
+    a_d = torch.from_numpy(a).to(dev)
+    anchor = args.anchor or tuning.autotune_anchor(a, m)
+    print(f"autotuned anchor interval {anchor}", file=sys.stderr, flush=True)
+    out = {}
+
+    def timed(name, ptype, precision, length, init):
+        prof = torch.full((length,), init, dtype=torch.int64 if ptype == "1NN_INDEX"
+                          else torch.int32, device=dev)
+        params = C.make_params(m, ptype, precision=precision, anchor_interval=anchor,
+                               matrix_height=H, matrix_width=H)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        C.join_device(a_d.data_ptr(), n, 0, 0, params, prof.data_ptr())
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": dt, "pairs_per_s": pairs / dt,
+                     "sweep_ms": C.context(0).timing()["sweep_ms"]}
+        print(f"{name}: {dt:.1f} s, {pairs / dt:.4g} pairs/s", file=sys.stderr, flush=True)
+        return prof.cpu().numpy()
+
+    init_nn = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
+    init_mat = int(C.init_profile("MATRIX_SUMMARY", 1, 1, 1)[0].view(np.int32))
+    p32 = timed("mixed_1nn_index", "1NN_INDEX", "mixed", ns, init_nn).view(np.uint64)
+    msum = timed("mixed_matrix_summary", "MATRIX_SUMMARY", "mixed", H * H,
+                 init_mat).view(np.float32).reshape(H, H)
+    p64 = timed("fp64_1nn_index", "1NN_INDEX", "double", ns, init_nn).view(np.uint64)
+    c32, i32 = C.split_mp_entry(p32)
+    c64, i64 = C.split_mp_entry(p64)
+    valid = c64 > -1
+    dcorr = float(np.max(np.abs(c32[valid].astype(np.float64) - c64[valid])))
+    agree = float(np.mean(i32[valid] == i64[valid]))
+    # summary rows against the profile: cell row r covers rows [r*rpc, (r+1)*rpc)
+    rpc = -(-ns // H)
+    prof_max = np.full(H, -2.0, dtype=np.float32)
+    for r in range(H):
+        seg = c32[r * rpc:(r + 1) * rpc]
+        if seg.size:
+            prof_max[r] = seg.max()
+    prof_max = np.where(prof_max > 0.0, prof_max, np.float32(-2.0))  # cells keep values > 0
+    row_max = np.max(np.where(np.isnan(msum), -2.0, msum), axis=1)
+    summary_rows_equal = bool(np.array_equal(row_max, prof_max))
+    # FP64 profile spot check with direct FP64 dot products
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for r in rng.choice(np.nonzero(valid)[0], 100, replace=False):
+        x = a[r:r + m] - a[r:r + m].mean()
+        j = int(i64[r])
+        y = a[j:j + m] - a[j:j + m].mean()
+        worst = max(worst, abs(float(np.float32(x @ y / np.sqrt((x @ x) * (y @ y)))) - c64[r]))
+    line = {
+        "metric": "self-join pairs/sec (n=2^25, m=4096, float32 input): mixed form",
+        "value": out["mixed_1nn_index"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": out["mixed_1nn_index"]["seconds"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "self-join of a float32 random walk, n=33554432 (PCG64 seed 5), "
+                               "m=4096, 1NN_INDEX + MATRIX_SUMMARY 1024x1024, mixed form",
+                   "n": n, "window": m, "pairs": pairs, "anchor_interval": anchor},
+        "joins": out,
+        "mixed_max_abs_dcorr_vs_fp64": dcorr, "mixed_index_agreement_vs_fp64": agree,
+        "summary_row_max_equals_profile": summary_rows_equal,
+        "fp64_spot_check_max_abs_err_100_rows": worst,
+    }
+    print(json.dumps(line))
+
+
 def calibrate(steps=3, nbytes=1 << 31):
     """A plain device copy of ``nbytes`` (elementwise kernel: read nbytes,
     write nbytes), run ``steps`` times, so that FETCH_SIZE / WRITE_SIZE
@@ -123,6 +229,8 @@ def main():
                     help="which forms to time (profiling runs time one form)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the copy kernel used to scale HBM counters")
+    ap.add_argument("--workload", choices=("selfjoin-n4194304-m1024", "selfjoin-n33554432-m4096"),
+                    default="selfjoin-n4194304-m1024")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the join on a single "
@@ -130,6 +238,9 @@ def main():
         sys.exit(2)
     if args.calibrate:
         calibrate()
+        return
+    if args.workload == "selfjoin-n33554432-m4096":
+        large_selfjoin(args)
         return
 
     import torch

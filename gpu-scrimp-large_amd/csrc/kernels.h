@@ -86,6 +86,8 @@ struct SweepArgs {
   double *sumX, *sumY;   // SUM_THRESH profiles
   uint32_t *mat;         // MATRIX_SUMMARY cells (order-preserving uint32)
   int64_t mat_w, rpc, cpc;
+  int32_t mat_mirror;    // rows and columns on one cell grid (self-join,
+                         // rpc == cpc): a transposed cell mirrors a direct one
   double thr;
   unsigned int *counter; // work counter, zeroed before launch
 };

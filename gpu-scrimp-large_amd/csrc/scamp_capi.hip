@@ -208,6 +208,7 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   a.nbands = nbands; a.m = m; a.anchor = anchor;
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
+  a.mat_mirror = p.X == p.Y && rpc == cpc;
   a.counter = counter;
 
   const int grid = (int)std::min<int64_t>(nbands, (int64_t)ctx->num_cus * 16);

@@ -107,20 +107,30 @@ struct Sweep {
   const int64_t d0;      // first diagonal of this lane
   // FP64 form: cov[k] is the covariance of diagonal d0+k at the current row.
   // FP32 form: the covariance is hi[k] + cov[k]; hi[k] is its leading part
-  // and cov[k] collects the row increments of the current block on top of
+  // and cov[k] collects the row increments of the last few rows on top of
   // the low part, so FP32 rounding hits the small partial sum every row and
-  // the large leading part only once per block (compensated, by Fast2Sum).
+  // the large leading part only once per kRenormRows rows (compensated, by
+  // Fast2Sum).  The partial's own rounding grows with its size, which is
+  // what sets kRenormRows: large increments (a high-amplitude stretch of one
+  // series) leave an absolute error behind that later low-norm windows
+  // magnify.
   static constexpr bool kComp = sizeof(DT) == 4;
+  static constexpr int kRenormRows = 4;
   DT cov[K];
   DT hi[K];
+  // SUM_THRESH rows: this lane's partial sum of each row of the current block
+  double rsum[K];
   Slot<DT> S[K];         // column slots, indexed by column mod K
   // MATRIX_SUMMARY state
+  const bool mirror;     // transposed cells mirror direct ones: tra unused
+  DT blk0, blk1;         // fast blocks: running max of the lane's pairs
   CellAcc dir, tra;
   int64_t dir_row_cell, dir_row_pos;  // uniform: row cell and row mod rpc
   int64_t tra_col_cell, tra_col_pos;  // uniform: column cell and row mod cpc
 
   __device__ Sweep(const SweepArgs &args, int ln, int64_t band_d0)
-      : a(args), lane(ln), D0(band_d0), d0(band_d0 + (int64_t)ln * K) {}
+      : a(args), lane(ln), D0(band_d0), d0(band_d0 + (int64_t)ln * K),
+        mirror(COLS && ROWS && args.mat_mirror) {}
 
   __device__ __forceinline__ bool diag_valid(int64_t d) const {
     return d >= a.d_lo && d < a.d_hi;
@@ -250,9 +260,9 @@ struct Sweep {
     }
   }
 
-  // FP32 form, end of a block: folds the block's partial sum into the
+  // FP32 form, every kRenormRows rows: folds the partial sum into the
   // leading part; the rounding error of that addition (exact by Fast2Sum
-  // while |hi| >= |partial|) seeds the next block's partial sum.
+  // while |hi| >= |partial|) seeds the next partial sum.
   __device__ __forceinline__ void renormalise() {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -273,8 +283,11 @@ struct Sweep {
 
   // ------------------------------------------------------ matrix summary
   __device__ __forceinline__ void cell_flush(double v, int64_t r, int64_t c) {
-    if (v > a.thr)
-      atomicMax(&a.mat[r * a.mat_w + c], enc_corr((float)v));
+    if (v > a.thr) {
+      const uint32_t e = enc_corr((float)v);
+      atomicMax(&a.mat[r * a.mat_w + c], e);
+      if (mirror) atomicMax(&a.mat[c * a.mat_w + r], e);
+    }
   }
   __device__ __forceinline__ void cells_begin(int64_t i) {
     const int64_t col0 = i + d0;
@@ -285,7 +298,7 @@ struct Sweep {
       dir_row_cell = i / a.rpc;
       dir_row_pos = i - dir_row_cell * a.rpc;
     }
-    if (ROWS) {  // transposed cells (column / rpc, row i / cpc)
+    if (ROWS && !mirror) {  // transposed cells (column / rpc, row i / cpc)
       tra.lo = tra.hi = nan_of<double>();
       tra.lo_cell = col0 / a.rpc;
       tra.pos = col0 - tra.lo_cell * a.rpc;
@@ -299,7 +312,7 @@ struct Sweep {
       cell_flush(dir.hi, dir_row_cell, dir.lo_cell + 1);
       dir.lo = dir.hi = nan_of<double>();
     }
-    if (ROWS) {
+    if (ROWS && !mirror) {
       cell_flush(tra.lo, tra.lo_cell, tra_col_cell);
       cell_flush(tra.hi, tra.lo_cell + 1, tra_col_cell);
       tra.lo = tra.hi = nan_of<double>();
@@ -313,7 +326,7 @@ struct Sweep {
       dir.lo = dir.hi = nan_of<double>();
       dir_row_cell += 1;
     }
-    if (ROWS && tra_col_pos == 0 && !first) {
+    if (ROWS && !mirror && tra_col_pos == 0 && !first) {
       cell_flush(tra.lo, tra.lo_cell, tra_col_cell);
       cell_flush(tra.hi, tra.lo_cell + 1, tra_col_cell);
       tra.lo = tra.hi = nan_of<double>();
@@ -331,7 +344,7 @@ struct Sweep {
         dir.hi = fmax(dir.hi, in_lo ? nan_of<double>() : v);
       }
     }
-    if (ROWS) {
+    if (ROWS && !mirror) {
       const int64_t kb = a.rpc - tra.pos;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -354,7 +367,7 @@ struct Sweep {
       }
       if (++dir_row_pos == a.rpc) dir_row_pos = 0;
     }
-    if (ROWS) {
+    if (ROWS && !mirror) {
       if (++tra.pos == a.rpc) {
         cell_flush(tra.lo, tra.lo_cell, tra_col_cell);
         tra.lo = tra.hi;
@@ -363,6 +376,43 @@ struct Sweep {
         tra.pos = 0;
       }
       if (++tra_col_pos == a.cpc) tra_col_pos = 0;
+    }
+  }
+
+  // Fast blocks: when no cell boundary falls inside any lane's part of the
+  // next K rows (columns first .. first + 2K - 2 of the lane, rows i_b ..
+  // i_b + K - 1), every pair of a lane goes to the lane's current cell, so
+  // the block keeps one running max per lane and folds it in at the end.
+  // Wave-uniform decision; blocks that touch a boundary take the per-pair
+  // path above.
+  __device__ __forceinline__ bool cells_block_fast() const {
+    if (COLS && ROWS && !mirror) return false;
+    bool ok;
+    if (COLS)
+      ok = dir.pos + 2 * K - 2 < a.cpc && dir_row_pos + K <= a.rpc;
+    else
+      ok = tra.pos + 2 * K - 2 < a.rpc && tra_col_pos + K <= a.cpc;
+    return __builtin_amdgcn_ballot_w64(!ok) == 0;
+  }
+  __device__ __forceinline__ void cells_fast_accumulate(const DT (&c)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; k += 2) {
+      blk0 = fmax(blk0, c[k]);
+      blk1 = fmax(blk1, c[k + 1]);
+    }
+  }
+  __device__ __forceinline__ void cells_fast_end() {
+    const double v = (double)fmax(blk0, blk1);
+    if (COLS) {
+      dir.lo = fmax(dir.lo, v);
+      dir.pos += K;
+      dir_row_pos += K;
+      if (dir_row_pos == a.rpc) dir_row_pos = 0;
+    } else {
+      tra.lo = fmax(tra.lo, v);
+      tra.pos += K;
+      tra_col_pos += K;
+      if (tra_col_pos == a.cpc) tra_col_pos = 0;
     }
   }
 
@@ -391,7 +441,7 @@ struct Sweep {
   // ------------------------------------------------------------ one row
   // Row i = i_b + r of a block whose first row satisfies (i_b + D0) % K == 0,
   // so diagonal k of every lane crosses the column in slot (r + k) % K.
-  template <bool MASKED>
+  template <bool MASKED, bool FAST>
   __device__ __forceinline__ void row(int r, int64_t i, const Terms<DT> *s_row) {
     const Terms<DT> rv = s_row[r];
     const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
@@ -433,19 +483,27 @@ struct Sweep {
         }
       }
     } else if (KIND == kSum) {
-      double rs = 0.0;
+      // a row with no correlation above the threshold anywhere in the wave
+      // costs one compare per pair
+      uint64_t hit = 0;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const double v = c[k] > (DT)a.thr ? (double)c[k] : 0.0;
-        if (ROWS) rs += v;
-        if (COLS) S[(r + k) % K].acc += v;
-      }
-      if (ROWS && __builtin_amdgcn_ballot_w64(rs != 0.0)) {
-        rs = wave_sum(rs);
-        if (lane == 0) atomicAdd(a.sumY + i, rs);
+      for (int k = 0; k < K; ++k)
+        hit |= __builtin_amdgcn_ballot_w64(c[k] > (DT)a.thr);
+      if (hit) {
+        double rs = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double v = c[k] > (DT)a.thr ? (double)c[k] : 0.0;
+          if (ROWS) rs += v;
+          if (COLS) S[(r + k) % K].acc += v;
+        }
+        if (ROWS) rsum[r] = rs;
       }
     } else if (KIND == kMatrix) {
-      cells_accumulate(c);
+      if (FAST)
+        cells_fast_accumulate(c);
+      else
+        cells_accumulate(c);
     } else {  // kMatrixSmall: cells narrower than a lane's 8 columns
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -464,7 +522,7 @@ struct Sweep {
       cov[k] = fma(s.df, rdg, cov[k]);
       cov[k] = fma(rdf, s.dg, cov[k]);
     }
-    if (KIND == kMatrix) cells_row_end();
+    if (KIND == kMatrix && !FAST) cells_row_end();
   }
 
   // Slot r now holds the column that leaves this lane (column i + d0): shift
@@ -487,13 +545,43 @@ struct Sweep {
   // Flushes what a lane still holds when its segment ends after row i.
   __device__ __forceinline__ void segment_end(int r, int64_t i) {
     if (KIND == kSum && COLS) {
+      // slot j holds diagonal (j - r) mod K (indexing by slot keeps the
+      // slot array in registers)
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const Slot<DT> &s = S[(r + k) % K];
-        if (s.acc != 0.0) atomicAdd(a.sumX + i + d0 + k, s.acc);
+      for (int j = 0; j < K; ++j) {
+        const int64_t k = (j - r + K) % K;
+        if (S[j].acc != 0.0) atomicAdd(a.sumX + i + d0 + k, S[j].acc);
       }
     }
     if (KIND == kMatrix) cells_flush_all();
+  }
+
+  // SUM_THRESH rows: adds the wave-wide total of rsum[r] to row i_b + r for
+  // r < nrows.  Reduce-scatter butterfly: after the xor-32/16/8 exchanges
+  // lane L holds a partial of row L >> 3 summed over 8 lanes, and the
+  // xor-4/2/1 steps complete it; lanes 0, 8, ..., 56 then add one row each.
+  __device__ __forceinline__ void flush_row_sums(int64_t i_b, int64_t nrows) {
+    static_assert(K == 8, "the butterfly below reduces 8 rows");
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    double v4[4], v2[2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double keep = b5 ? rsum[4 + j] : rsum[j];
+      const double send = b5 ? rsum[j] : rsum[4 + j];
+      v4[j] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double keep = b4 ? v4[2 + j] : v4[j];
+      const double send = b4 ? v4[j] : v4[2 + j];
+      v2[j] = keep + __shfl_xor(send, 16);
+    }
+    double v = (b3 ? v2[1] : v2[0]) + __shfl_xor(b3 ? v2[0] : v2[1], 8);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 1);
+    const int row = lane >> 3;
+    if ((lane & 7) == 0 && row < nrows && v != 0.0) atomicAdd(a.sumY + i_b + row, v);
   }
 
   // One block of K rows starting at i_b.  `pre` holds this block's staged
@@ -502,24 +590,49 @@ struct Sweep {
   __device__ __forceinline__ bool block(int64_t i_b, int64_t re, Pre &pre,
                                         Terms<DT> *s_row, Terms<DT> *s_col,
                                         bool first) {
+    if (KIND == kMatrix && !MASKED && cells_block_fast())
+      return block_rows<false, true>(i_b, re, pre, s_row, s_col, first);
+    return block_rows<MASKED, false>(i_b, re, pre, s_row, s_col, first);
+  }
+
+  template <bool MASKED, bool FAST>
+  __device__ __forceinline__ bool block_rows(int64_t i_b, int64_t re, Pre &pre,
+                                             Terms<DT> *s_row, Terms<DT> *s_col,
+                                             bool first) {
     publish(pre, s_row, s_col);
     prefetch(pre, i_b + K);
     // keep the look-ahead loads where they are issued: the scheduler would
     // otherwise sink them next to their first use a block later
     __builtin_amdgcn_sched_barrier(0);
+    if (KIND == kSum && ROWS) {
+#pragma unroll
+      for (int r = 0; r < K; ++r) rsum[r] = 0.0;
+    }
+    if (KIND == kMatrix && FAST) {
+      cells_row_start(first);
+      blk0 = blk1 = nan_of<DT>();
+    }
+    bool ended = false;
 #pragma unroll
     for (int r = 0; r < K; ++r) {
       const int64_t i = i_b + r;
-      if (MASKED && i >= re) return true;
-      if (KIND == kMatrix) cells_row_start(first && r == 0);
-      row<MASKED>(r, i, s_row);
+      if (MASKED && i >= re) {
+        ended = true;
+        break;
+      }
+      if (KIND == kMatrix && !FAST) cells_row_start(first && r == 0);
+      row<MASKED, FAST>(r, i, s_row);
       if (MASKED && i + 1 == re) {
         segment_end(r, i);
-        return true;
+        ended = true;
+        break;
       }
       shift(r, i, s_col);
+      if (kComp && r % kRenormRows == kRenormRows - 1) renormalise();
     }
-    if (kComp) renormalise();
+    if (KIND == kSum && ROWS) flush_row_sums(i_b, re - i_b);
+    if (KIND == kMatrix && FAST) cells_fast_end();
+    if (ended) return true;
     __builtin_amdgcn_wave_barrier();
     return false;
   }

@@ -206,7 +206,14 @@ def test_sum_thresh_matches_oracle_and_brute_force(thr):
 @pytest.mark.parametrize("na,nb,m,H,W,thr", [(1200, None, 32, 5, 10, 0.1),
                                              (1500, 1100, 50, 7, 9, 0.0),
                                              (400, None, 20, 50, 50, 0.0),
-                                             (3000, 2000, 64, 10, 5, 0.125)])
+                                             (3000, 2000, 64, 10, 5, 0.125),
+                                             # wide cells: whole blocks of rows
+                                             # fall in one cell (square self-join
+                                             # grid: mirrored cells; AB: each
+                                             # pass on its own side)
+                                             (6000, None, 64, 10, 10, 0.0),
+                                             (9000, None, 100, 7, 7, 0.3),
+                                             (5000, 7000, 50, 6, 4, 0.1)])
 def test_matrix_summary_is_exact_max_pool(na, nb, m, H, W, thr):
     a = random_walk(na, na + 3)
     b = None if nb is None else random_walk(nb, nb + 4)
