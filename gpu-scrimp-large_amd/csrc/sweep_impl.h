@@ -116,6 +116,12 @@ struct Sweep {
   // magnify.
   static constexpr bool kComp = sizeof(DT) == 4;
   static constexpr int kRenormRows = 4;
+  // Rows per staged block.  The FP32 1NN kinds take two slot cycles per
+  // block, which halves the per-block work (staging, threshold decoding,
+  // loop-carried register copies); measured 7 % faster at n = 2^22,
+  // m = 1024.  The FP64 form keeps one cycle: 16 rows raise its register
+  // count past the 3-waves-per-SIMD limit and it measured 3 % slower.
+  static constexpr int kRows = KIND == kNN && sizeof(DT) == 4 ? 2 * K : K;
   DT cov[K];
   DT hi[K];
   // SUM_THRESH rows: this lane's partial sum of each row of the current block
@@ -159,7 +165,7 @@ struct Sweep {
 
   // Terms staged a block ahead: lane l fetches row ib+l (the rows of the
   // block) and column ib+D0+kBand+l (the column entering the band at lane 63
-  // on row ib+l); lanes >= K fetch clamped / padded entries nobody reads, so
+  // on row ib+l); lanes >= kRows fetch clamped / padded entries nobody reads, so
   // the code has no divergent branch.
   struct Pre {
     DT rdf, rdg, rin;
@@ -441,9 +447,11 @@ struct Sweep {
   // ------------------------------------------------------------ one row
   // Row i = i_b + r of a block whose first row satisfies (i_b + D0) % K == 0,
   // so diagonal k of every lane crosses the column in slot (r + k) % K.
+  // r: slot cycle position (i - i_b) mod K; rr: row of the block (i - i_b).
   template <bool MASKED, bool FAST>
-  __device__ __forceinline__ void row(int r, int64_t i, const Terms<DT> *s_row) {
-    const Terms<DT> rv = s_row[r];
+  __device__ __forceinline__ void row(int r, int rr, int64_t i,
+                                      const Terms<DT> *s_row) {
+    const Terms<DT> rv = s_row[rr];
     const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
     DT c[K];
 #pragma unroll
@@ -528,10 +536,10 @@ struct Sweep {
   // Slot r now holds the column that leaves this lane (column i + d0): shift
   // every lane's slot r one lane down.  Lane 0's column leaves the band (its
   // partial sum is flushed); lane 63 takes the entering column.
-  __device__ __forceinline__ void shift(int r, int64_t i,
+  __device__ __forceinline__ void shift(int r, int rr, int64_t i,
                                         const Terms<DT> *s_col) {
     Slot<DT> &s = S[r];
-    const Terms<DT> e = s_col[r];
+    const Terms<DT> e = s_col[rr];
     if (KIND == kSum && COLS) {
       if (lane == 0 && s.acc != 0.0) atomicAdd(a.sumX + i + D0, s.acc);
       s.acc = shl1(s.acc, 0.0);
@@ -584,7 +592,7 @@ struct Sweep {
     if ((lane & 7) == 0 && row < nrows && v != 0.0) atomicAdd(a.sumY + i_b + row, v);
   }
 
-  // One block of K rows starting at i_b.  `pre` holds this block's staged
+  // One block of kRows rows starting at i_b.  `pre` holds this block's staged
   // terms on entry and the next block's on exit.
   template <bool MASKED>
   __device__ __forceinline__ bool block(int64_t i_b, int64_t re, Pre &pre,
@@ -600,7 +608,7 @@ struct Sweep {
                                              Terms<DT> *s_row, Terms<DT> *s_col,
                                              bool first) {
     publish(pre, s_row, s_col);
-    prefetch(pre, i_b + K);
+    prefetch(pre, i_b + kRows);
     // keep the look-ahead loads where they are issued: the scheduler would
     // otherwise sink them next to their first use a block later
     __builtin_amdgcn_sched_barrier(0);
@@ -613,22 +621,23 @@ struct Sweep {
       blk0 = blk1 = nan_of<DT>();
     }
     bool ended = false;
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      const int64_t i = i_b + r;
+#pragma clang loop unroll(full)
+    for (int rr = 0; rr < kRows; ++rr) {
+      const int r = rr % K;
+      const int64_t i = i_b + rr;
       if (MASKED && i >= re) {
         ended = true;
         break;
       }
       if (KIND == kMatrix && !FAST) cells_row_start(first && r == 0);
-      row<MASKED, FAST>(r, i, s_row);
+      row<MASKED, FAST>(r, rr, i, s_row);
       if (MASKED && i + 1 == re) {
         segment_end(r, i);
         ended = true;
         break;
       }
-      shift(r, i, s_col);
-      if (kComp && r % kRenormRows == kRenormRows - 1) renormalise();
+      shift(r, rr, i, s_col);
+      if (kComp && rr % kRenormRows == kRenormRows - 1) renormalise();
     }
     if (KIND == kSum && ROWS) flush_row_sums(i_b, re - i_b);
     if (KIND == kMatrix && FAST) cells_fast_end();
@@ -656,22 +665,22 @@ struct Sweep {
     // Fast region; the last row always falls in the masked tail below,
     // which flushes.
     int64_t to_anchor = a.anchor;
-    while (i + K < re && i + K <= fast_end) {
+    while (i + kRows < re && i + kRows <= fast_end) {
       if (sizeof(DT) == 4) {
         if (to_anchor <= 0) {
           anchor(i, s_ax, s_ay);
           to_anchor = a.anchor;
         }
-        to_anchor -= K;
+        to_anchor -= kRows;
       }
       block<false>(i, re, pre, s_row, s_col, first);
       first = false;
-      i += K;
+      i += kRows;
     }
     for (;;) {
       if (block<true>(i, re, pre, s_row, s_col, first)) return;
       first = false;
-      i += K;
+      i += kRows;
     }
   }
 };
