@@ -14,6 +14,9 @@ constexpr int kBand = kWave * kDiagsPerLane;  // diagonals per wave band
 // the sweep's look-ahead loads, the lanes of a partial last band and the
 // FP32 form's re-anchoring windows never leave the allocation.
 constexpr int64_t kColumnPad = 2 * kBand + 8 * kDiagsPerLane;
+// Longest series the sweep takes: (subsequences + padding) * 8 bytes must fit
+// the 32-bit byte offsets its loads and atomics use.
+constexpr int64_t kMaxSubsequences = (int64_t(1) << 29) - 2 * kColumnPad;
 
 __host__ __device__ inline uint32_t f32_bits(float f) {
   uint32_t b;

@@ -6,8 +6,13 @@
 // read as the prior best-so-far), throw SCAMPException on failure.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <mutex>
+#include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -246,4 +251,126 @@ double ConvertToEuclidean(double val, int window) {
 double CleanupPearson(double val) {
   if (val < -1) return NAN;
   return val;
+}
+
+// ---------------------------------------------------------------------------
+// Text formats of the command-line tool (src/scamp_utils.cpp:1-224).
+
+namespace {
+
+void write_value(std::ofstream &out, double v) {
+  out << std::setprecision(10) << v;
+}
+
+}  // namespace
+
+bool readFile(const std::string &filename, std::vector<double> &v) {
+  std::ifstream f(filename);
+  if (f.fail()) {
+    std::cout << "Unable to open" << filename
+              << "for reading, please make sure it exists" << std::endl;
+    return false;
+  }
+  // Token by token (blank lines are skipped); as in the reference reader
+  // (src/scamp_utils.cpp:28-74) a value is kept only when some character
+  // follows it, so a last value without a line end is dropped.
+  std::string tok;
+  while (f >> tok) {
+    double d;
+    try {
+      d = std::stod(tok);
+    } catch (const std::invalid_argument &) {
+      std::cout << tok[0] << std::endl
+                << "FATAL ERROR: invalid argument: Could not parse line number "
+                << v.size() + 1 << " from input file.\n";
+      std::exit(1);
+    } catch (const std::out_of_range &) {
+      std::cout << tok[0] << std::endl
+                << "FATAL ERROR: out of range: Could not parse line number "
+                << v.size() + 1 << " from input file.\n";
+      std::exit(1);
+    }
+    if (f.peek() == EOF) break;
+    v.push_back(d);
+  }
+  return true;
+}
+
+std::vector<int> ParseIntList(const std::string &s) {
+  std::vector<int> out;
+  if (s.empty()) return out;
+  std::stringstream ss(s);
+  std::string part;
+  while (std::getline(ss, part, ',')) out.push_back(std::stoi(part));
+  return out;
+}
+
+SCAMP::SCAMPPrecisionType GetPrecisionType(bool ultrap, bool doublep,
+                                           bool mixedp, bool singlep) {
+  if (ultrap) return SCAMP::PRECISION_ULTRA;
+  if (doublep) return SCAMP::PRECISION_DOUBLE;
+  if (mixedp) return SCAMP::PRECISION_MIXED;
+  if (singlep) return SCAMP::PRECISION_SINGLE;
+  return SCAMP::PRECISION_INVALID;
+}
+
+SCAMP::SCAMPProfileType ParseProfileType(const std::string &s) {
+  if (s == "1NN_INDEX") return SCAMP::PROFILE_TYPE_1NN_INDEX;
+  if (s == "SUM_THRESH") return SCAMP::PROFILE_TYPE_SUM_THRESH;
+  if (s == "1NN") return SCAMP::PROFILE_TYPE_1NN;
+  if (s == "ALL_NEIGHBORS") return SCAMP::PROFILE_TYPE_APPROX_ALL_NEIGHBORS;
+  if (s == "MATRIX_SUMMARY") return SCAMP::PROFILE_TYPE_MATRIX_SUMMARY;
+  return SCAMP::PROFILE_TYPE_INVALID;
+}
+
+bool WriteProfileToFile(const std::string &mp, const std::string &mpi,
+                        SCAMP::Profile &p, bool output_pearson, int window,
+                        int matrix_width, int matrix_height) {
+  (void)matrix_height;
+  auto convert = [&](double corr) {
+    return output_pearson ? CleanupPearson(corr) : ConvertToEuclidean(corr, window);
+  };
+  switch (p.type) {
+    case SCAMP::PROFILE_TYPE_1NN_INDEX: {
+      std::ofstream out(mp), iout(mpi);
+      for (const uint64_t u : p.data[0].uint64_value) {
+        SCAMP::mp_entry e;
+        e.ulong = u;
+        write_value(out, convert(e.floats[0]));
+        out << std::endl;
+        iout << (e.floats[0] < -1 ? -1 : (int)e.ints[1]) << std::endl;
+      }
+      return true;
+    }
+    case SCAMP::PROFILE_TYPE_1NN: {
+      std::ofstream out(mp);
+      for (const float f : p.data[0].float_value) {
+        write_value(out, convert(f));
+        out << std::endl;
+      }
+      return true;
+    }
+    case SCAMP::PROFILE_TYPE_SUM_THRESH: {
+      std::ofstream out(mp);
+      for (const double d : p.data[0].double_value) {
+        write_value(out, d);
+        out << std::endl;
+      }
+      return true;
+    }
+    case SCAMP::PROFILE_TYPE_MATRIX_SUMMARY: {
+      // rows of matrix_width values, each followed by a space
+      std::ofstream out(mp);
+      const std::vector<float> &m = p.data[0].float_value;
+      for (size_t i = 0; i < m.size(); ++i) {
+        if (i > 0 && i % (size_t)matrix_width == 0) out << std::endl;
+        write_value(out, convert(m[i]));
+        out << " ";
+      }
+      out << std::endl;
+      return true;
+    }
+    default:
+      return false;
+  }
 }

@@ -237,6 +237,12 @@ const char *validate(const scamp_hip_params *p, int64_t na, int64_t nb) {
   if (na < p->window || (p->has_b && nb < p->window))
     return "Error: Input time series must be at least 'subesequence window "
            "size' in length";
+  // The sweep addresses a column's terms and profile key with 32-bit byte
+  // offsets (8-byte entries, padded arrays).
+  if (na - p->window + 1 > kMaxSubsequences ||
+      (p->has_b && nb - p->window + 1 > kMaxSubsequences))
+    return "Error: time series too long for this engine (at most 536868736 "
+           "subsequences per series)";
   return nullptr;
 }
 

@@ -140,3 +140,24 @@ std::string getSCAMPErrorString(SCAMPError_t err);
 bool InitProfileMemory(SCAMP::SCAMPArgs *args);
 double ConvertToEuclidean(double val, int window);
 double CleanupPearson(double val);
+
+// The file formats of SCAMP's command-line tool (src/scamp_utils.cpp:1-224).
+// readFile: one value per whitespace-separated token (std::stod, so "NaN"
+// and "inf" parse; blank lines are skipped); a value is kept only when some
+// character follows it, so a final value without a line end is dropped, as
+// in the reference; an unparsable token prints a FATAL ERROR naming its
+// position and exits the process with status 1, as the reference does.
+// Returns false (after a message) when the file cannot be opened.
+bool readFile(const std::string &filename, std::vector<double> &v);
+// Comma-separated integers ("0,1,3"); empty -> empty list.
+std::vector<int> ParseIntList(const std::string &s);
+SCAMP::SCAMPPrecisionType GetPrecisionType(bool ultrap, bool doublep,
+                                           bool mixedp, bool singlep);
+SCAMP::SCAMPProfileType ParseProfileType(const std::string &s);
+// Writes profile p as text: 1NN_INDEX -> distances (or correlations with
+// output_pearson) to `mp` and indices (-1 = no match) to `mpi`, one per line;
+// 1NN -> `mp`; SUM_THRESH -> sums; MATRIX_SUMMARY -> matrix_height lines of
+// matrix_width space-separated values.  Values with 10 significant digits.
+bool WriteProfileToFile(const std::string &mp, const std::string &mpi,
+                        SCAMP::Profile &p, bool output_pearson, int window,
+                        int matrix_width, int matrix_height);

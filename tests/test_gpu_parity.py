@@ -331,3 +331,13 @@ def test_invalid_arguments_return_codes():
         C.join(a[:5], None, C.make_params(10))
     with pytest.raises(C.ScampError, match="matrix height"):
         C.join(a, None, C.make_params(10, "MATRIX_SUMMARY", matrix_height=0))
+    # lengths past the sweep's 32-bit offsets are refused before any buffer
+    # is read (the claimed length here is far beyond the 500-sample array)
+    import ctypes
+    prof = C.init_profile("1NN_INDEX", a.size)
+    ctx = C.context(0)
+    rc = ctx._lib.scamp_hip_join(ctx.handle, C._ptr(a), ctypes.c_size_t(1 << 30), None, 0,
+                                 ctypes.byref(C.make_params(10)), C._ptr(prof), None)
+    assert rc == 16
+    with pytest.raises(C.ScampError, match="too long"):
+        C.check(rc)
