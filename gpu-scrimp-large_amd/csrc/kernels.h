@@ -61,8 +61,19 @@ hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
                                    const int32_t *index, const double *grad_p,
                                    double *grad_t, hipStream_t s);
 
+// ---- ALL_NEIGHBORS list conversions --------------------------------------
+// prior matches (scamp_hip_match[n * k], corr > -2 = a match) -> key lists
+// (entries not above `init` become `init`) and each list's minimum in keys[j]
+hipError_t launch_matches_to_knn(const void *matches, int64_t n, int k,
+                                 uint64_t init, uint64_t *lists, uint64_t *keys,
+                                 hipStream_t s);
+// key lists -> matches sorted best first (entries not above `init` become
+// empty: corr -2, row 0xFFFFFFFF); sorts the lists in place
+hipError_t launch_knn_to_matches(uint64_t *lists, int64_t n, int k, uint64_t init,
+                                 void *matches, hipStream_t s);
+
 // ---- the diagonal sweep (sweep_kernel.hip) --------------------------------
-enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3 };
+enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3, kKnn = 4 };
 
 struct SweepArgs {
   // column side X / row side Y update terms and inverse norms (FP64 or FP32,
@@ -88,6 +99,10 @@ struct SweepArgs {
   int64_t mat_w, rpc, cpc;
   int32_t mat_mirror;    // rows and columns on one cell grid (self-join,
                          // rpc == cpc): a transposed cell mirrors a direct one
+  // ALL_NEIGHBORS: per subsequence a list of knn_k order-preserving keys (the
+  // k best matches so far; keyX / keyY then hold each list's minimum)
+  uint64_t *knnX, *knnY;
+  int32_t knn_k;
   double thr;
   unsigned int *counter; // work counter, zeroed before launch
 };

@@ -193,6 +193,62 @@ __global__ void keys_to_mp_kernel(uint64_t *p, int64_t n) {
   p[i] = (static_cast<uint64_t>(key_index(k)) << 32) | f32_bits(key_corr(k));
 }
 
+// ALL_NEIGHBORS: prior matches -> key lists and list minima.
+struct MatchRec {
+  float corr;
+  uint32_t row;
+  uint32_t col;
+};
+
+__global__ void matches_to_knn_kernel(const MatchRec *m, int64_t n, int k,
+                                      uint64_t init, uint64_t *lists,
+                                      uint64_t *keys) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint64_t mn = ~0ull;
+  for (int t = 0; t < k; ++t) {
+    const MatchRec r = m[j * k + t];
+    uint64_t key = init;
+    if (r.corr > -2.0f) {
+      const uint64_t kk = make_key(r.corr, r.row);
+      if (kk > init) key = kk;
+    }
+    // a prior entry repeated in the list counts once
+    for (int u = 0; u < t; ++u)
+      if (lists[j * k + u] == key) key = init;
+    lists[j * k + t] = key;
+    mn = key < mn ? key : mn;
+  }
+  keys[j] = mn;
+}
+
+// Key lists -> matches, best first (selection sort of each short list in
+// place; k is the number of neighbours asked for, typically a handful).
+__global__ void knn_to_matches_kernel(uint64_t *lists, int64_t n, int k,
+                                      uint64_t init, MatchRec *m) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint64_t *l = lists + j * k;
+  for (int t = 0; t < k; ++t) {
+    int best = t;
+    for (int u = t + 1; u < k; ++u)
+      if (l[u] > l[best]) best = u;
+    const uint64_t key = l[best];
+    l[best] = l[t];
+    l[t] = key;
+    MatchRec r;
+    r.col = static_cast<uint32_t>(j);
+    if (key > init) {
+      r.corr = key_corr(key);
+      r.row = key_index(key);
+    } else {
+      r.corr = -2.0f;
+      r.row = 0xFFFFFFFFu;
+    }
+    m[j * k + t] = r;
+  }
+}
+
 __global__ void f32_to_keys_kernel(const float *src, uint64_t *dst, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) dst[i] = make_key(src[i], 0xFFFFFFFFu);
@@ -269,6 +325,18 @@ hipError_t launch_mp_to_keys(uint64_t *p, int64_t n, hipStream_t s) {
 }
 hipError_t launch_keys_to_mp(uint64_t *p, int64_t n, hipStream_t s) {
   SCAMP_SIMPLE_LAUNCH(keys_to_mp_kernel, n, p, n);
+}
+hipError_t launch_matches_to_knn(const void *matches, int64_t n, int k,
+                                 uint64_t init, uint64_t *lists, uint64_t *keys,
+                                 hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(matches_to_knn_kernel, n,
+                      static_cast<const MatchRec *>(matches), n, k, init, lists,
+                      keys);
+}
+hipError_t launch_knn_to_matches(uint64_t *lists, int64_t n, int k, uint64_t init,
+                                 void *matches, hipStream_t s) {
+  SCAMP_SIMPLE_LAUNCH(knn_to_matches_kernel, n, lists, n, k, init,
+                      static_cast<MatchRec *>(matches));
 }
 hipError_t launch_f32_to_keys(const float *src, uint64_t *dst, int64_t n,
                               hipStream_t s) {

@@ -5,11 +5,14 @@
 //   abjoin(a, b, m, **kw)         -> (float32[n_sub(a)], int32[n_sub(a)] into b)
 //   selfjoin_sum / abjoin_sum     -> float64[n_sub(a)]
 //   selfjoin_matrix / abjoin_matrix -> float32[mheight, mwidth]
+//   selfjoin_knn(a, m, k) / abjoin_knn(a, b, m, k)
+//                                 -> [(col, row, distance), ...]: for each
+//                                    subsequence col of a, its k best matches
+//                                    above the threshold, worst first
 //   gpu_supported()               -> bool
 // Keywords: verbose, precision ('single'|'mixed'|'double'|'ultra'), pearson,
-// gpus, threads, plus threshold (sum/matrix) and mheight/mwidth (matrix);
-// anything else raises ValueError, as in the reference.  selfjoin_knn /
-// abjoin_knn (ALL_NEIGHBORS) are not part of this engine and raise.
+// gpus, threads, plus threshold (sum/knn/matrix) and mheight/mwidth (matrix);
+// anything else raises ValueError, as in the reference.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -18,6 +21,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "scamp.hpp"
@@ -64,12 +68,15 @@ bool key_ok(const std::string &key, SCAMP::SCAMPProfileType type) {
                                                  "pearson", "gpus", "threads"};
   static const std::set<std::string> sum_thresh = {
       "verbose", "precision", "pearson", "gpus", "threads", "threshold"};
+  static const std::set<std::string> knn = {
+      "verbose", "precision", "pearson", "gpus", "threads", "threshold"};
   static const std::set<std::string> matrix = {
       "verbose", "precision", "pearson", "gpus",
       "threads", "threshold", "mheight", "mwidth"};
   switch (type) {
     case SCAMP::PROFILE_TYPE_1NN_INDEX: return nn_index.count(key) == 1;
     case SCAMP::PROFILE_TYPE_SUM_THRESH: return sum_thresh.count(key) == 1;
+    case SCAMP::PROFILE_TYPE_APPROX_ALL_NEIGHBORS: return knn.count(key) == 1;
     case SCAMP::PROFILE_TYPE_MATRIX_SUMMARY: return matrix.count(key) == 1;
     default: return false;
   }
@@ -236,10 +243,48 @@ py::array_t<float> abjoin_matrix(const Series &a, const Series &b, int m,
   return matrix_out(args, pearson, m);
 }
 
-py::object knn_unsupported(py::args, py::kwargs) {
-  throw std::runtime_error(
-      "ALL_NEIGHBORS (approximate KNN) profiles are not supported by this "
-      "engine");
+// SplitProfileKNN (python/SCAMP_python.cpp:31-50): each subsequence's kept
+// matches in heap order (worst first), as (col, row, distance) tuples.
+std::vector<std::tuple<int64_t, int64_t, float>> split_knn(SCAMP::SCAMPArgs &args,
+                                                           bool pearson) {
+  std::vector<std::tuple<int64_t, int64_t, float>> out;
+  for (auto &q : args.profile_a.data[0].match_value) {
+    while (!q.empty()) {
+      const SCAMP::SCAMPmatch &e = q.top();
+      const float d = pearson ? (float)CleanupPearson(e.corr)
+                              : (float)ConvertToEuclidean(e.corr, (int)args.window);
+      out.emplace_back((int64_t)e.col, (int64_t)e.row, d);
+      q.pop();
+    }
+  }
+  return out;
+}
+
+std::vector<std::tuple<int64_t, int64_t, float>> selfjoin_knn(const Series &a, int m,
+                                                              int k,
+                                                              const py::kwargs &kw) {
+  auto args = default_args(SCAMP::PROFILE_TYPE_APPROX_ALL_NEIGHBORS);
+  args.timeseries_a = to_vector(a);
+  args.timeseries_b = args.timeseries_a;
+  args.window = m;
+  args.max_matches_per_column = k;
+  bool pearson = run(&args, kw);
+  return split_knn(args, pearson);
+}
+
+std::vector<std::tuple<int64_t, int64_t, float>> abjoin_knn(const Series &a,
+                                                            const Series &b, int m,
+                                                            int k,
+                                                            const py::kwargs &kw) {
+  auto args = default_args(SCAMP::PROFILE_TYPE_APPROX_ALL_NEIGHBORS);
+  args.timeseries_a = to_vector(a);
+  args.timeseries_b = to_vector(b);
+  args.window = m;
+  args.has_b = true;
+  args.computing_rows = false;
+  args.max_matches_per_column = k;
+  bool pearson = run(&args, kw);
+  return split_knn(args, pearson);
 }
 
 }  // namespace
@@ -261,7 +306,12 @@ PYBIND11_MODULE(pyscamp, m) {
         "Max-pooled distance matrix of shape (mheight, mwidth)");
   m.def("abjoin_matrix", &abjoin_matrix, py::arg("a"), py::arg("b"),
         py::arg("m"), "Max-pooled AB distance matrix of shape (mheight, mwidth)");
-  m.def("selfjoin_knn", &knn_unsupported);
-  m.def("abjoin_knn", &knn_unsupported);
+  m.def("selfjoin_knn", &selfjoin_knn, py::arg("a"), py::arg("m"), py::arg("k"),
+        "For each subsequence of a, its k best matches in a (list of (col, row, "
+        "distance))");
+  m.def("abjoin_knn", &abjoin_knn, py::arg("a"), py::arg("b"), py::arg("m"),
+        py::arg("k"),
+        "For each subsequence of a, its k best matches in b (list of (col, row, "
+        "distance))");
   m.attr("__version__") = "mi355x";
 }

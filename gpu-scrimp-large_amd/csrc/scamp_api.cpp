@@ -4,6 +4,7 @@
 // the arguments, run the join, leave the profile(s) in args->profile_a / _b
 // (which the caller pre-allocated, e.g. with InitProfileMemory, and which are
 // read as the prior best-so-far), throw SCAMPException on failure.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,6 +47,7 @@ void Profile::Alloc(size_t size, int64_t matrix_height, int64_t matrix_width,
       break;
     case PROFILE_TYPE_APPROX_ALL_NEIGHBORS:
       data.emplace_back();
+      data[0].match_value.resize(size);
       thresholds.resize(size, default_thresh);
       break;
     case PROFILE_TYPE_MATRIX_SUMMARY:
@@ -142,7 +144,40 @@ size_t profile_length(const Profile &p, SCAMPProfileType t) {
   switch (t) {
     case PROFILE_TYPE_1NN_INDEX: return p.data[0].uint64_value.size();
     case PROFILE_TYPE_SUM_THRESH: return p.data[0].double_value.size();
+    case PROFILE_TYPE_APPROX_ALL_NEIGHBORS: return p.data[0].match_value.size();
     default: return p.data[0].float_value.size();
+  }
+}
+
+// ALL_NEIGHBORS: the per-subsequence queues <-> the C ABI's k-entry arrays.
+std::vector<scamp_hip_match> queues_to_matches(Profile &p, size_t n, int64_t k) {
+  std::vector<scamp_hip_match> out(n * k);
+  for (size_t j = 0; j < n; ++j) {
+    auto q = p.data[0].match_value[j];  // copy: the caller's queue stays
+    std::vector<SCAMPmatch> v;
+    while (!q.empty()) {
+      v.push_back(q.top());
+      q.pop();
+    }
+    // best first, at most k of them
+    std::vector<scamp_hip_match> row(k, scamp_hip_match{-2.0f, 0xFFFFFFFFu, (uint32_t)j});
+    int64_t t = 0;
+    for (auto it = v.rbegin(); it != v.rend() && t < k; ++it, ++t)
+      row[t] = scamp_hip_match{it->corr, it->row, (uint32_t)j};
+    std::copy(row.begin(), row.end(), out.begin() + j * k);
+  }
+  return out;
+}
+
+void matches_to_queues(const std::vector<scamp_hip_match> &m, Profile &p, size_t n,
+                       int64_t k) {
+  for (size_t j = 0; j < n; ++j) {
+    auto &q = p.data[0].match_value[j];
+    q = std::priority_queue<SCAMPmatch, std::vector<SCAMPmatch>, compareMatch>();
+    for (int64_t t = 0; t < k; ++t) {
+      const scamp_hip_match &e = m[j * k + t];
+      if (e.corr > -2.0f) q.emplace(e.corr, e.row, (uint32_t)j);
+    }
   }
 }
 
@@ -164,7 +199,8 @@ void do_SCAMP(SCAMPArgs *args, const std::vector<int> &devices,
   }
   const SCAMPProfileType t = args->profile_type;
   if (t != PROFILE_TYPE_1NN_INDEX && t != PROFILE_TYPE_1NN &&
-      t != PROFILE_TYPE_SUM_THRESH && t != PROFILE_TYPE_MATRIX_SUMMARY) {
+      t != PROFILE_TYPE_SUM_THRESH && t != PROFILE_TYPE_MATRIX_SUMMARY &&
+      t != PROFILE_TYPE_APPROX_ALL_NEIGHBORS) {
     throw SCAMPException("ERROR " +
                          getSCAMPErrorString(SCAMP_FUNCTIONALITY_UNIMPLEMENTED) +
                          " executing tile");
@@ -205,12 +241,29 @@ void do_SCAMP(SCAMPArgs *args, const std::vector<int> &devices,
   p.matrix_width = args->matrix_width;
   p.max_tile_size = (int64_t)args->max_tile_size;
   p.anchor_interval = args->anchor_interval;
+  p.max_matches_per_column = args->max_matches_per_column;
 
-  int rc = scamp_hip_join(ctx, args->timeseries_a.data(), na,
-                          args->has_b ? args->timeseries_b.data() : nullptr,
-                          args->has_b ? nb : 0, &p,
-                          profile_buffer(args->profile_a, t),
-                          sep ? profile_buffer(args->profile_b, t) : nullptr);
+  int rc;
+  if (t == PROFILE_TYPE_APPROX_ALL_NEIGHBORS) {
+    const int64_t k = args->max_matches_per_column;
+    if (k < 1 || k > 256)
+      throw SCAMPException("Error: max_matches_per_column must be between 1 and 256");
+    std::vector<scamp_hip_match> ma = queues_to_matches(args->profile_a, na - m + 1, k);
+    std::vector<scamp_hip_match> mb;
+    if (sep) mb = queues_to_matches(args->profile_b, nb - m + 1, k);
+    rc = scamp_hip_join(ctx, args->timeseries_a.data(), na,
+                        args->has_b ? args->timeseries_b.data() : nullptr,
+                        args->has_b ? nb : 0, &p, ma.data(), sep ? mb.data() : nullptr);
+    if (rc == SCAMP_HIP_OK) {
+      matches_to_queues(ma, args->profile_a, na - m + 1, k);
+      if (sep) matches_to_queues(mb, args->profile_b, nb - m + 1, k);
+    }
+  } else {
+    rc = scamp_hip_join(ctx, args->timeseries_a.data(), na,
+                        args->has_b ? args->timeseries_b.data() : nullptr,
+                        args->has_b ? nb : 0, &p, profile_buffer(args->profile_a, t),
+                        sep ? profile_buffer(args->profile_b, t) : nullptr);
+  }
   if (rc != SCAMP_HIP_OK) {
     char buf[512];
     scamp_hip_last_error(buf, sizeof(buf));
@@ -355,6 +408,23 @@ bool WriteProfileToFile(const std::string &mp, const std::string &mpi,
       for (const double d : p.data[0].double_value) {
         write_value(out, d);
         out << std::endl;
+      }
+      return true;
+    }
+    case SCAMP::PROFILE_TYPE_APPROX_ALL_NEIGHBORS: {
+      // "col row value" per match, each subsequence's matches best first
+      std::ofstream out(mp);
+      for (auto q : p.data[0].match_value) {
+        std::vector<SCAMP::SCAMPmatch> v;
+        while (!q.empty()) {
+          v.push_back(q.top());
+          q.pop();
+        }
+        for (auto it = v.rbegin(); it != v.rend(); ++it) {
+          out << it->col << " " << it->row << " ";
+          write_value(out, convert(it->corr));
+          out << std::endl;
+        }
       }
       return true;
     }

@@ -25,6 +25,9 @@
 
 using namespace scamp_amd;
 
+// the prep kernels' MatchRec mirrors this layout
+static_assert(sizeof(scamp_hip_match) == 12, "scamp_hip_match is 12 bytes");
+
 // Calls on one context run one after another (each holds `mu`); separate
 // contexts, on the same device or not, run independently.
 struct scamp_hip_ctx {
@@ -135,12 +138,13 @@ struct Pass {
   bool cols, rows;
   uint64_t *keyX, *keyY;
   double *sumX, *sumY;
+  uint64_t *knnX, *knnY;  // ALL_NEIGHBORS lists
 };
 
 // Runs one pass (one diagonal range of one orientation).
 void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
               bool fp32, SweepKind kind, uint32_t *mat, int64_t mat_w,
-              int64_t rpc, int64_t cpc, double thr, int64_t anchor,
+              int64_t rpc, int64_t cpc, double thr, int64_t anchor, int knn_k,
               float *sweep_ms) {
   hipStream_t s = ctx->stream;
   const int64_t nX = p.X->ns, nY = p.Y->ns;
@@ -207,6 +211,7 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi; a.d_base = d_base;
   a.nbands = nbands; a.m = m; a.anchor = anchor;
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
+  a.knnX = p.knnX; a.knnY = p.knnY; a.knn_k = knn_k;
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
   a.mat_mirror = p.X == p.Y && rpc == cpc;
   a.counter = counter;
@@ -239,6 +244,9 @@ const char *validate(const scamp_hip_params *p, int64_t na, int64_t nb) {
            "size' in length";
   // The sweep addresses a column's terms and profile key with 32-bit byte
   // offsets (8-byte entries, padded arrays).
+  if (p->profile_type == SCAMP_HIP_ALL_NEIGHBORS &&
+      (p->max_matches_per_column < 1 || p->max_matches_per_column > 256))
+    return "Error: max_matches_per_column must be between 1 and 256";
   if (na - p->window + 1 > kMaxSubsequences ||
       (p->has_b && nb - p->window + 1 > kMaxSubsequences))
     return "Error: time series too long for this engine (at most 536868736 "
@@ -257,7 +265,8 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   }
   const int pt = p->profile_type;
   if (pt != SCAMP_HIP_1NN_INDEX && pt != SCAMP_HIP_1NN &&
-      pt != SCAMP_HIP_SUM_THRESH && pt != SCAMP_HIP_MATRIX_SUMMARY)
+      pt != SCAMP_HIP_SUM_THRESH && pt != SCAMP_HIP_MATRIX_SUMMARY &&
+      pt != SCAMP_HIP_ALL_NEIGHBORS)
     return fail(SCAMP_HIP_UNIMPLEMENTED,
                 "profile type not supported by this engine");
   int prec = p->precision;
@@ -289,9 +298,25 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   // Working profiles (padded) initialised from the caller's prior.
   uint64_t *keyA = nullptr, *keyB = nullptr;
   double *sumA = nullptr, *sumB = nullptr;
+  uint64_t *knnA = nullptr, *knnB = nullptr;
   uint32_t *mat = nullptr;
   const int64_t H = p->matrix_height, W = p->matrix_width;
-  if (pt == SCAMP_HIP_1NN_INDEX || pt == SCAMP_HIP_1NN) {
+  const int knn_k = (int)p->max_matches_per_column;
+  // ALL_NEIGHBORS: a match enters a list when its key beats `knn_init`,
+  // i.e. when its FP32 correlation is above the threshold
+  const uint64_t knn_init = make_key((float)p->distance_threshold, 0u);
+  if (pt == SCAMP_HIP_ALL_NEIGHBORS) {
+    keyA = ar.alloc<uint64_t>(nsA + kColumnPad);
+    HIP_CHECK(launch_fill_u64(keyA + nsA, kColumnPad, knn_init, s));
+    knnA = ar.alloc<uint64_t>(nsA * knn_k);
+    HIP_CHECK(launch_matches_to_knn(d_prof_a, nsA, knn_k, knn_init, knnA, keyA, s));
+    if (sep_rows) {
+      keyB = ar.alloc<uint64_t>(nsB + kColumnPad);
+      HIP_CHECK(launch_fill_u64(keyB + nsB, kColumnPad, knn_init, s));
+      knnB = ar.alloc<uint64_t>(nsB * knn_k);
+      HIP_CHECK(launch_matches_to_knn(d_prof_b, nsB, knn_k, knn_init, knnB, keyB, s));
+    }
+  } else if (pt == SCAMP_HIP_1NN_INDEX || pt == SCAMP_HIP_1NN) {
     const uint64_t init = make_key(-2.0f, 0xFFFFFFFFu);
     keyA = ar.alloc<uint64_t>(nsA + kColumnPad);
     HIP_CHECK(launch_fill_u64(keyA + nsA, kColumnPad, init, s));
@@ -336,6 +361,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   SweepKind kind = kNN;
   int64_t rpc = 1, cpc = 1;
   if (pt == SCAMP_HIP_SUM_THRESH) kind = kSum;
+  if (pt == SCAMP_HIP_ALL_NEIGHBORS) kind = kKnn;
   if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
     cpc = (nsA + W - 1) / W;
     rpc = (nsB + H - 1) / H;
@@ -346,7 +372,8 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   if (self) {
     for (auto &r : cut(e, nsA, 0, 0)) {
       Pass ps{&A, &A, r.first, r.second, cols_out, rows_out, keyA,
-              sep_rows ? keyB : keyA, sumA, sep_rows ? sumB : sumA};
+              sep_rows ? keyB : keyA, sumA, sep_rows ? sumB : sumA,
+              knnA, sep_rows ? knnB : knnA};
       if (pt == SCAMP_HIP_MATRIX_SUMMARY) ps.cols = ps.rows = true;
       passes.push_back(ps);
     }
@@ -359,7 +386,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     for (auto &r : al ? cut(0, nsA, -e - delta, e - delta)
                       : cut(0, nsA, 0, 0)) {
       Pass ps{&A, &B, r.first, r.second, cols_out, sep_rows, keyA, keyB,
-              sumA, sumB};
+              sumA, sumB, knnA, knnB};
       if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
         ps.cols = true;
         ps.rows = false;
@@ -370,7 +397,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     for (auto &r : al ? cut(1, nsB, delta - e, delta + e)
                       : cut(1, nsB, 0, 0)) {
       Pass ps{&B, &A, r.first, r.second, sep_rows, cols_out, keyB, keyA,
-              sumB, sumA};
+              sumB, sumA, knnB, knnA};
       if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
         ps.cols = false;
         ps.rows = true;
@@ -381,7 +408,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   float sweep_ms = 0;
   for (const Pass &ps : passes)
     run_pass(ctx, ar, ps, m, fp32, kind, mat, W, rpc, cpc,
-             p->distance_threshold, anchor, &sweep_ms);
+             p->distance_threshold, anchor, knn_k, &sweep_ms);
 
   // Back to SCAMP's layouts.
   if (pt == SCAMP_HIP_1NN_INDEX) {
@@ -396,6 +423,10 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   } else if (pt == SCAMP_HIP_1NN) {
     HIP_CHECK(launch_keys_to_f32(keyA, (float *)d_prof_a, nsA, s));
     if (sep_rows) HIP_CHECK(launch_keys_to_f32(keyB, (float *)d_prof_b, nsB, s));
+  } else if (pt == SCAMP_HIP_ALL_NEIGHBORS) {
+    HIP_CHECK(launch_knn_to_matches(knnA, nsA, knn_k, knn_init, d_prof_a, s));
+    if (sep_rows)
+      HIP_CHECK(launch_knn_to_matches(knnB, nsB, knn_k, knn_init, d_prof_b, s));
   } else if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
     HIP_CHECK(launch_enc_to_f32(reinterpret_cast<float *>(mat), H * W, s));
     HIP_CHECK(hipMemcpyAsync(d_prof_a, mat, sizeof(float) * H * W,
@@ -422,6 +453,7 @@ size_t profile_elem_bytes(int pt) {
   switch (pt) {
     case SCAMP_HIP_1NN_INDEX: return sizeof(uint64_t);
     case SCAMP_HIP_SUM_THRESH: return sizeof(double);
+    case SCAMP_HIP_ALL_NEIGHBORS: return sizeof(scamp_hip_match);
     default: return sizeof(float);
   }
 }
@@ -513,10 +545,13 @@ int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
   const int64_t m = p->window;
   const size_t eb = profile_elem_bytes(p->profile_type);
   const bool mat = p->profile_type == SCAMP_HIP_MATRIX_SUMMARY;
+  // entries per subsequence (ALL_NEIGHBORS: k matches)
+  const size_t per = p->profile_type == SCAMP_HIP_ALL_NEIGHBORS
+                         ? (size_t)p->max_matches_per_column : 1;
   const size_t len_a =
-      mat ? (size_t)(p->matrix_height * p->matrix_width) : na - m + 1;
+      mat ? (size_t)(p->matrix_height * p->matrix_width) : (na - m + 1) * per;
   const bool sep = !mat && p->computing_rows && p->keep_rows_separate;
-  const size_t len_b = sep ? (p->has_b ? nb : na) - m + 1 : 0;
+  const size_t len_b = sep ? ((p->has_b ? nb : na) - m + 1) * per : 0;
   if (sep && !profile_b) return fail(SCAMP_HIP_INVALID_ARGS, "missing profile b");
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);

@@ -8,6 +8,7 @@ hipError_t launch_sweep_sum(const SweepArgs &, bool, bool, bool, int, hipStream_
 hipError_t launch_sweep_matrix(const SweepArgs &, bool, bool, bool, int, hipStream_t);
 hipError_t launch_sweep_matrix_small(const SweepArgs &, bool, bool, bool, int,
                                      hipStream_t);
+hipError_t launch_sweep_knn(const SweepArgs &, bool, bool, bool, int, hipStream_t);
 
 hipError_t launch_sweep(const SweepArgs &a, bool fp32, SweepKind kind,
                         bool cols, bool rows, int grid, hipStream_t s) {
@@ -16,6 +17,7 @@ hipError_t launch_sweep(const SweepArgs &a, bool fp32, SweepKind kind,
     case kNN: return launch_sweep_nn(a, fp32, cols, rows, grid, s);
     case kSum: return launch_sweep_sum(a, fp32, cols, rows, grid, s);
     case kMatrix: return launch_sweep_matrix(a, fp32, cols, rows, grid, s);
+    case kKnn: return launch_sweep_knn(a, fp32, cols, rows, grid, s);
     default: return launch_sweep_matrix_small(a, fp32, cols, rows, grid, s);
   }
 }

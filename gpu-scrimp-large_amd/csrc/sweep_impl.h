@@ -115,6 +115,8 @@ struct Sweep {
   // series) leave an absolute error behind that later low-norm windows
   // magnify.
   static constexpr bool kComp = sizeof(DT) == 4;
+  // kinds that test each pair against per-row / per-column threshold keys
+  static constexpr bool kThr = KIND == kNN || KIND == kKnn;
   static constexpr int kRenormRows = 4;
   // Rows per staged block.  The FP32 1NN kinds take two slot cycles per
   // block, which halves the per-block work (staging, threshold decoding,
@@ -157,7 +159,7 @@ struct Sweep {
     s.df = ld<DT>(a.dfX, off);
     s.dg = ld<DT>(a.dgX, off);
     s.in = ld<DT>(a.inX, off);
-    if (KIND == kNN && COLS)
+    if (kThr && COLS)
       s.t = Thresh<DT>::from_key_hi(
           ld<uint32_t>(a.keyX, static_cast<uint32_t>(col) * 8u + 4u));
     if (KIND == kSum) s.acc = 0.0;
@@ -180,14 +182,14 @@ struct Sweep {
     p.rdf = ld<DT>(a.dfY, roff);
     p.rdg = ld<DT>(a.dgY, roff);
     p.rin = ld<DT>(a.inY, roff);
-    if (KIND == kNN && ROWS)
+    if (kThr && ROWS)
       p.rkey = ld<uint32_t>(a.keyY, static_cast<uint32_t>(ri) * 8u + 4u);
     const int64_t ci = ib + D0 + kBand + lane;
     const uint32_t coff = static_cast<uint32_t>(ci) * sizeof(DT);
     p.cdf = ld<DT>(a.dfX, coff);
     p.cdg = ld<DT>(a.dgX, coff);
     p.cin = ld<DT>(a.inX, coff);
-    if (KIND == kNN && COLS)
+    if (kThr && COLS)
       p.ckey = ld<uint32_t>(a.keyX, static_cast<uint32_t>(ci) * 8u + 4u);
   }
   __device__ __forceinline__ void publish(const Pre &p, Terms<DT> *s_row,
@@ -196,11 +198,11 @@ struct Sweep {
     r.df = p.rdf;
     r.dg = p.rdg;
     r.in = p.rin;
-    r.t = (KIND == kNN && ROWS) ? Thresh<DT>::from_key_hi(p.rkey) : (DT)0;
+    r.t = (kThr && ROWS) ? Thresh<DT>::from_key_hi(p.rkey) : (DT)0;
     c.df = p.cdf;
     c.dg = p.cdg;
     c.in = p.cin;
-    c.t = (KIND == kNN && COLS) ? Thresh<DT>::from_key_hi(p.ckey) : (DT)0;
+    c.t = (kThr && COLS) ? Thresh<DT>::from_key_hi(p.ckey) : (DT)0;
     s_row[lane] = r;
     s_col[lane] = c;
     __builtin_amdgcn_wave_barrier();
@@ -444,6 +446,94 @@ struct Sweep {
                 static_cast<unsigned long long>(key));
   }
 
+  // ------------------------------------------------- ALL_NEIGHBORS lists
+  // Inserts `key` into the k-entry list at `list` if it beats the list's
+  // minimum, by replacing that minimum with a compare-and-swap (retried
+  // when another insert changed the list first; every retry follows
+  // someone else's successful insert, so the loop ends).  Returns a lower
+  // bound of the list's minimum after the call.  Entries are read with
+  // device-coherent atomic loads so that a retry sees the newest values.
+  __device__ __forceinline__ static uint64_t knn_insert(uint64_t *list, int kk,
+                                                        uint64_t key) {
+    for (;;) {
+      uint64_t mn = ~0ull, second = ~0ull;
+      int slot = 0;
+      bool present = false;
+      for (int t = 0; t < kk; ++t) {
+        const uint64_t v = __hip_atomic_load(list + t, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        present |= v == key;  // a pair already in the list (from the prior)
+        if (v < mn) {
+          second = mn;
+          mn = v;
+          slot = t;
+        } else if (v < second) {
+          second = v;
+        }
+      }
+      if (key <= mn || present) return mn;
+      const unsigned long long old = atomicCAS(
+          reinterpret_cast<unsigned long long *>(list + slot),
+          static_cast<unsigned long long>(mn), static_cast<unsigned long long>(key));
+      if (old == mn) return key < second ? key : second;
+    }
+  }
+
+  // ALL_NEIGHBORS slow path of one row: the row's list takes the wave's best
+  // candidates (at most k, found by repeated wave maxima so that one lane
+  // inserts them); each column's list takes the lane's candidate for it.
+  __device__ __forceinline__ void knn_update(int r, const DT (&c)[K], int64_t i,
+                                             DT trow) {
+    const int kk = a.knn_k;
+    if (ROWS) {
+      uint32_t taken = 0;
+      uint64_t *list = a.knnY + i * kk;
+      for (int q = 0; q < kk; ++q) {
+        uint64_t best = 0;
+        int bk = -1;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!((taken >> k) & 1u) && c[k] >= trow) {
+            const uint64_t key =
+                make_key((float)c[k], static_cast<uint32_t>(i + d0 + k));
+            if (key > best) {
+              best = key;
+              bk = k;
+            }
+          }
+        }
+        const uint64_t wbest = wave_max_u64(best);
+        if (wbest == 0) break;
+        if (bk >= 0 && best == wbest) taken |= 1u << bk;  // keys are unique
+        uint64_t nm = 0;
+        if (lane == 0) {
+          nm = knn_insert(list, kk, wbest);
+          atomicMax(reinterpret_cast<unsigned long long *>(a.keyY + i),
+                    static_cast<unsigned long long>(nm));
+        }
+        // keys in one list are unique, so a candidate that did not go in
+        // left a minimum above it, and so would every smaller one
+        nm = __shfl(nm, 0);
+        if (nm > wbest) break;
+      }
+    }
+    if (COLS) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        Slot<DT> &s = S[(r + k) % K];
+        if (c[k] >= s.t) {
+          const int64_t j = i + d0 + k;
+          const uint64_t key = make_key((float)c[k], static_cast<uint32_t>(i));
+          const uint64_t nm = knn_insert(a.knnX + j * kk, kk, key);
+          atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + j),
+                    static_cast<unsigned long long>(nm));
+          const DT tn = Thresh<DT>::from_key_hi(static_cast<uint32_t>(nm >> 32));
+          s.t = s.t > tn ? s.t : tn;
+        }
+      }
+    }
+  }
+
   // ------------------------------------------------------------ one row
   // Row i = i_b + r of a block whose first row satisfies (i_b + D0) % K == 0,
   // so diagonal k of every lane crosses the column in slot (r + k) % K.
@@ -460,7 +550,7 @@ struct Sweep {
       c[k] = cv * S[(r + k) % K].in * rin;
       if (MASKED && i + d0 + k >= a.nX) c[k] = nan_of<DT>();
     }
-    if (KIND == kNN) {
+    if (kThr) {
       // One compare per test written straight into a wave mask (keeps the
       // compiler from merging the two tests into a NaN-canonicalising min).
       uint64_t hit = 0;
@@ -469,7 +559,9 @@ struct Sweep {
         if (COLS) hit |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].t);
         if (ROWS) hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
       }
-      if (hit) {
+      if (KIND == kKnn && hit) {
+        knn_update(r, c, i, trow);
+      } else if (hit) {
         if (ROWS) nn_row_update(c, i, trow);
         if (COLS) {
 #pragma unroll
@@ -547,7 +639,7 @@ struct Sweep {
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
     s.in = shl1(s.in, e.in);
-    if (KIND == kNN && COLS) s.t = shl1(s.t, e.t);
+    if (kThr && COLS) s.t = shl1(s.t, e.t);
   }
 
   // Flushes what a lane still holds when its segment ends after row i.

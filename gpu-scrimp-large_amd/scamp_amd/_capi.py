@@ -15,7 +15,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libscamp_hip.so")
 
 # include/scamp_hip.h enums
-PROFILE_TYPES = {"1NN_INDEX": 1, "SUM_THRESH": 2, "1NN": 6, "MATRIX_SUMMARY": 8}
+PROFILE_TYPES = {"1NN_INDEX": 1, "SUM_THRESH": 2, "1NN": 6, "ALL_NEIGHBORS": 7,
+                 "MATRIX_SUMMARY": 8}
+# scamp_hip_match
+MATCH_DTYPE = np.dtype([("corr", np.float32), ("row", np.uint32), ("col", np.uint32)])
 PRECISIONS = {"single": 1, "mixed": 2, "double": 3, "ultra": 4}
 STATUS = {0: "OK", 1: "FUNCTIONALITY_UNIMPLEMENTED", 2: "ILLEGAL_TYPE",
           3: "DEVICE_ERROR", 6: "DIM_INCOMPATIBLE", 16: "INVALID_ARGS",
@@ -47,6 +50,7 @@ class Params(ctypes.Structure):
         ("matrix_width", ctypes.c_int64),
         ("max_tile_size", ctypes.c_int64),
         ("anchor_interval", ctypes.c_int64),
+        ("max_matches_per_column", ctypes.c_int64),
     ]
 
 
@@ -152,16 +156,25 @@ def make_params(window, profile_type="1NN_INDEX", precision="double", has_b=Fals
                 computing_rows=True, computing_columns=True, keep_rows_separate=False,
                 is_aligned=False, silent_mode=True, distributed_start_row=-1,
                 distributed_start_col=-1, threshold=0.0, matrix_height=50,
-                matrix_width=50, max_tile_size=1 << 17, anchor_interval=0) -> Params:
+                matrix_width=50, max_tile_size=1 << 17, anchor_interval=0,
+                max_matches=5) -> Params:
     return Params(int(window), PROFILE_TYPES[profile_type], PRECISIONS[precision], int(has_b),
                   int(computing_rows), int(computing_columns), int(keep_rows_separate),
                   int(is_aligned), int(silent_mode), int(distributed_start_row),
                   int(distributed_start_col), float(threshold), int(matrix_height),
-                  int(matrix_width), int(max_tile_size), int(anchor_interval))
+                  int(matrix_width), int(max_tile_size), int(anchor_interval),
+                  int(max_matches))
 
 
-def init_profile(profile_type, n, matrix_height=50, matrix_width=50):
-    """Profile::Alloc initial values (src/common.cpp:283-319)."""
+def init_profile(profile_type, n, matrix_height=50, matrix_width=50, max_matches=5):
+    """Profile::Alloc initial values (src/common.cpp:283-319); ALL_NEIGHBORS:
+    n * max_matches empty scamp_hip_match entries."""
+    if profile_type == "ALL_NEIGHBORS":
+        out = np.zeros(n * max_matches, dtype=MATCH_DTYPE)
+        out["corr"] = -2.0
+        out["row"] = 0xFFFFFFFF
+        out["col"] = np.repeat(np.arange(n, dtype=np.uint32), max_matches)
+        return out
     if profile_type == "1NN_INDEX":
         e = np.uint64((0xFFFFFFFF << 32) | int(np.float32(-2.0).view(np.uint32)))
         return np.full(n, e, dtype=np.uint64)
@@ -172,6 +185,16 @@ def init_profile(profile_type, n, matrix_height=50, matrix_width=50):
     if profile_type == "MATRIX_SUMMARY":
         return np.full(matrix_height * matrix_width, -2.0, dtype=np.float32)
     raise ValueError(profile_type)
+
+
+def knn_lists(matches, k):
+    """scamp_hip_match[n * k] -> (corr float32[n, k], row int64[n, k]), best
+    first per subsequence; empty entries are NaN / -1."""
+    m = np.asarray(matches).reshape(-1, k)
+    corr = m["corr"].astype(np.float32)
+    row = m["row"].astype(np.int64)
+    empty = corr <= -2
+    return np.where(empty, np.float32(np.nan), corr), np.where(empty, -1, row)
 
 
 def split_mp_entry(p):
@@ -194,11 +217,12 @@ def join(a, b, params: Params, profile_a=None, profile_b=None, device=0):
         la, lb = params.matrix_height * params.matrix_width, 0
     else:
         la, lb = max(na - m + 1, 1), max(nb - m + 1, 1)
+    k = params.max_matches_per_column
     if profile_a is None:
-        profile_a = init_profile(pt, la, params.matrix_height, params.matrix_width)
+        profile_a = init_profile(pt, la, params.matrix_height, params.matrix_width, k)
     sep = params.computing_rows and params.keep_rows_separate and pt != "MATRIX_SUMMARY"
     if sep and profile_b is None:
-        profile_b = init_profile(pt, lb)
+        profile_b = init_profile(pt, lb, max_matches=k)
     ctx = context(device)
     check(ctx._lib.scamp_hip_join(ctx.handle, _ptr(a), na, _ptr(b), nb if b is not None else 0,
                                   ctypes.byref(params), _ptr(profile_a),

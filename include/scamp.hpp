@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <exception>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -71,11 +72,37 @@ typedef union {
   uint64_t ulong;
 } mp_entry;
 
+// One ALL_NEIGHBORS match (src/common.h:76-96): `row` is the matching
+// subsequence of subsequence `col`.  Ordered by column, then best first.
+struct SCAMPmatch {
+  SCAMPmatch() : corr(-2), row(0), col(0) {}
+  SCAMPmatch(float d, uint32_t r, uint32_t c) : corr(d), row(r), col(c) {}
+  bool operator<(const SCAMPmatch &other) const {
+    if (col == other.col) return corr > other.corr;
+    return col < other.col;
+  }
+  float corr;
+  uint32_t row;
+  uint32_t col;
+};
+
+// Heap order of a column's matches: the top is the worst kept match.
+class compareMatch {
+ public:
+  bool operator()(const SCAMPmatch &x1, const SCAMPmatch &x2) {
+    return x1.corr > x2.corr;
+  }
+};
+
 struct ProfileData {
   std::vector<uint32_t> uint32_value;
   std::vector<uint64_t> uint64_value;
   std::vector<float> float_value;
   std::vector<double> double_value;
+  // ALL_NEIGHBORS: per subsequence, its kept matches (at most
+  // max_matches_per_column)
+  std::vector<std::priority_queue<SCAMPmatch, std::vector<SCAMPmatch>, compareMatch>>
+      match_value;
 };
 
 class Profile {

@@ -14,6 +14,10 @@
  *   1NN            float correlation, "no match" = -2
  *   SUM_THRESH     double sum of correlations above the threshold, init 0
  *   MATRIX_SUMMARY float[height * width] row-major, init -2
+ *   ALL_NEIGHBORS  scamp_hip_match[n_sub * k]: for subsequence j its k best
+ *                  matches, entries j*k .. j*k+k-1, best first; an empty
+ *                  entry has corr -2 and row 0xFFFFFFFF (k =
+ *                  max_matches_per_column)
  * Profiles are read as the prior best-so-far and updated in place, as
  * SCAMP's tiles do (src/tile.cpp:322-389).
  */
@@ -45,8 +49,17 @@ enum {
   SCAMP_HIP_1NN_INDEX = 1,
   SCAMP_HIP_SUM_THRESH = 2,
   SCAMP_HIP_1NN = 6,
+  SCAMP_HIP_ALL_NEIGHBORS = 7,
   SCAMP_HIP_MATRIX_SUMMARY = 8
 };
+
+/* One ALL_NEIGHBORS match: Pearson correlation of subsequence `col` with
+ * subsequence `row` of the other series (or of the same one). */
+typedef struct {
+  float corr;
+  uint32_t row;
+  uint32_t col;
+} scamp_hip_match;
 
 /* Precisions: the SCAMPPrecisionType values (src/common.h:52-58).
  * DOUBLE (and ULTRA, which maps to it) = all-FP64 recurrence;
@@ -80,6 +93,7 @@ typedef struct {
   int64_t max_tile_size;       /* validated as SCAMP does; the sweep is untiled */
   int64_t anchor_interval;     /* MIXED only: rows between FP64 re-anchors,
                                   0 = library default (8192) */
+  int64_t max_matches_per_column; /* ALL_NEIGHBORS: k, 1..256 */
 } scamp_hip_params;
 
 /* Replaces: the per-worker device state of SCAMP's Tile/ExecInfo

@@ -71,8 +71,9 @@ def test_pyscamp_keyword_validation():
         mp.selfjoin_matrix(a, 32, mheight=0)
     with pytest.raises(ValueError, match="cpu worker threads"):
         mp.selfjoin(a, 32, threads=-1)
-    with pytest.raises(RuntimeError, match="ALL_NEIGHBORS"):
-        mp.selfjoin_knn(a, 32, 5)
+    # ALL_NEIGHBORS takes the threshold keyword but no matrix size
+    with pytest.raises(ValueError, match="unknown argument: mheight"):
+        mp.selfjoin_knn(a, 32, 5, mheight=4)
     assert mp.gpu_supported() is True
 
 
@@ -93,3 +94,28 @@ def test_pyscamp_without_gpu_fails_loudly():
     import pyscamp as mp
     with pytest.raises(RuntimeError, match="no HIP device|no compute_resources"):
         mp.selfjoin(np.random.default_rng(1).standard_normal(300), 16)
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """The Python binding's Params and match records have the C layouts
+    (compiled against include/scamp_hip.h with the host C compiler)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "scamp_hip.h"\n'
+                   'int main(void) { printf("%zu %zu %zu %zu\\n", sizeof(scamp_hip_params), '
+                   'offsetof(scamp_hip_params, anchor_interval), '
+                   'offsetof(scamp_hip_params, max_matches_per_column), '
+                   'sizeof(scamp_hip_match)); return 0; }\n')
+    exe = tmp_path / "sz"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    size, off_anchor, off_k, match = (int(x) for x in out)
+    assert size == ctypes.sizeof(C.Params)
+    assert off_anchor == C.Params.anchor_interval.offset
+    assert off_k == C.Params.max_matches_per_column.offset
+    assert match == C.MATCH_DTYPE.itemsize == 12
