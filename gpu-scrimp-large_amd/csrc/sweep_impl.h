@@ -118,12 +118,12 @@ struct Sweep {
   // kinds that test each pair against per-row / per-column threshold keys
   static constexpr bool kThr = KIND == kNN || KIND == kKnn;
   static constexpr int kRenormRows = 4;
-  // Rows per staged block.  The FP32 1NN kinds take two slot cycles per
-  // block, which halves the per-block work (staging, threshold decoding,
-  // loop-carried register copies); measured 7 % faster at n = 2^22,
-  // m = 1024.  The FP64 form keeps one cycle: 16 rows raise its register
-  // count past the 3-waves-per-SIMD limit and it measured 3 % slower.
-  static constexpr int kRows = KIND == kNN && sizeof(DT) == 4 ? 2 * K : K;
+  // Rows per staged block.  The 1NN kinds take two slot cycles per block,
+  // which halves the per-block work (staging, threshold decoding,
+  // loop-carried register copies); together with the occupancy targets on
+  // sweep_kernel this measured 4.4 % (FP64) and 7.5 % (FP32) faster at
+  // n = 2^22, m = 1024 than one cycle per block.
+  static constexpr int kRows = KIND == kNN ? 2 * K : K;
   DT cov[K];
   DT hi[K];
   // SUM_THRESH rows: this lane's partial sum of each row of the current block
@@ -777,8 +777,19 @@ struct Sweep {
   }
 };
 
+// Occupancy targets of the 1NN kinds: 3 waves per SIMD for FP64 (<= 168
+// VGPRs) and 4 for FP32 (<= 128).  With 16-row blocks the compiler would
+// otherwise use more registers for look-ahead and drop a wave; the few
+// spills the targets cost are cheaper than the lost wave (measured).
+template <typename DT, int KIND>
+constexpr int sweep_waves_per_eu() {
+  return KIND == kNN ? (sizeof(DT) == 4 ? 4 : 3) : 1;
+}
+
 template <typename DT, int KIND, bool COLS, bool ROWS>
-__global__ __launch_bounds__(kWave) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(kWave)
+__attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<DT, KIND>())))
+void sweep_kernel(SweepArgs a) {
   __shared__ Terms<DT> s_row[kWave];
   __shared__ Terms<DT> s_col[kWave];
   // FP32 form only: anchor staging (unreferenced, hence unallocated, in FP64)
