@@ -58,6 +58,17 @@ struct HipFailure {
     if (e_ != hipSuccess) throw HipFailure{e_, #call};     \
   } while (0)
 
+// A HIP event owned by one scope (destroyed on every exit path).
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { HIP_CHECK(hipEventCreate(&e)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event &) = delete;
+  Event &operator=(const Event &) = delete;
+};
+
 // Device allocations owned by one join; freed on scope exit.
 struct DevArena {
   std::vector<void *> ptrs;
@@ -217,18 +228,14 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   a.counter = counter;
 
   const int grid = (int)std::min<int64_t>(nbands, (int64_t)ctx->num_cus * 16);
-  hipEvent_t e0, e1;
-  HIP_CHECK(hipEventCreate(&e0));
-  HIP_CHECK(hipEventCreate(&e1));
-  HIP_CHECK(hipEventRecord(e0, s));
+  Event e0, e1;
+  HIP_CHECK(hipEventRecord(e0.e, s));
   HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s));
-  HIP_CHECK(hipEventRecord(e1, s));
-  HIP_CHECK(hipEventSynchronize(e1));
+  HIP_CHECK(hipEventRecord(e1.e, s));
+  HIP_CHECK(hipEventSynchronize(e1.e));
   float ms = 0;
-  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_CHECK(hipEventElapsedTime(&ms, e0.e, e1.e));
   *sweep_ms += ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
 }
 
 const char *validate(const scamp_hip_params *p, int64_t na, int64_t nb) {
@@ -282,11 +289,8 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
 
   hipStream_t s = ctx->stream;
   DevArena ar;
-  hipEvent_t ev_start, ev_prep, ev_end;
-  HIP_CHECK(hipEventCreate(&ev_start));
-  HIP_CHECK(hipEventCreate(&ev_prep));
-  HIP_CHECK(hipEventCreate(&ev_end));
-  HIP_CHECK(hipEventRecord(ev_start, s));
+  Event ev_start, ev_prep, ev_end;
+  HIP_CHECK(hipEventRecord(ev_start.e, s));
 
   SeriesStats A = make_stats(ar, d_a, na, m, fp32, s);
   SeriesStats B = self ? A : make_stats(ar, d_b, nb, m, fp32, s);
@@ -348,7 +352,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
                              hipMemcpyDeviceToDevice, s));
     HIP_CHECK(launch_f32_to_enc(reinterpret_cast<float *>(mat), H * W, s));
   }
-  HIP_CHECK(hipEventRecord(ev_prep, s));
+  HIP_CHECK(hipEventRecord(ev_prep.e, s));
 
   // Passes.  Self-join: diagonals [m/4, nsA) of the upper triangle, each pair
   // updating its row and its column in the same profile (src/tile.cpp:12-38
@@ -432,16 +436,13 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     HIP_CHECK(hipMemcpyAsync(d_prof_a, mat, sizeof(float) * H * W,
                              hipMemcpyDeviceToDevice, s));
   }
-  HIP_CHECK(hipEventRecord(ev_end, s));
-  HIP_CHECK(hipEventSynchronize(ev_end));
+  HIP_CHECK(hipEventRecord(ev_end.e, s));
+  HIP_CHECK(hipEventSynchronize(ev_end.e));
   float total = 0;
-  HIP_CHECK(hipEventElapsedTime(&total, ev_start, ev_end));
+  HIP_CHECK(hipEventElapsedTime(&total, ev_start.e, ev_end.e));
   ctx->total_ms = total;
   ctx->sweep_ms = sweep_ms;
   ctx->prep_ms = total - sweep_ms;
-  (void)hipEventDestroy(ev_start);
-  (void)hipEventDestroy(ev_prep);
-  (void)hipEventDestroy(ev_end);
   if (!p->silent_mode)
     std::printf("scamp_hip: join of %lld x %lld subsequences in %.3f ms "
                 "(sweep %.3f ms)\n",
