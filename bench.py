@@ -6,6 +6,7 @@ form's error against FP64.
                     [--anchor R] [--form both|fp64|mixed]
     python bench.py --calibrate     # copy kernel of known size, for --pmc scaling
     python bench.py --workload selfjoin-n33554432-m4096   # the large self-join
+    python bench.py --workload batch-64x8192-m128         # differentiable batched join
 
 One step = one complete self-join (profile reset, statistics, exact seeds,
 the diagonal sweep, profile conversion) of the workload below, with the input
@@ -203,6 +204,66 @@ def large_selfjoin(args):
     print(json.dumps(line))
 
 
+def batch_selfjoin(args):
+    """The differentiable batched self-join: 64 float32 random walks of 8192
+    samples (PCG64 seeds 100..163), m = 128.  One step = forward (one batched
+    FP64 self-join launch of all 64 series, 1NN_INDEX as correlations) plus
+    backward of L = sum_i (1 - P_i) (the HIP gradient kernel).  Also timed:
+    the forward alone, and the same 64 joins issued one series at a time."""
+    import torch
+    from scamp_amd import _capi as C
+    from scamp_amd.autograd import matrix_profile
+    B, n, m = 64, 8192, 128
+    X = np.stack([np.cumsum(np.random.Generator(np.random.PCG64(100 + b))
+                            .standard_normal(n)).astype(np.float32) for b in range(B)])
+    dev = torch.device("cuda:0")
+    T = torch.tensor(X, device=dev, requires_grad=True)
+    pairs = B * pair_count(n, m)
+
+    def step():
+        T.grad = None
+        P, _ = matrix_profile(T, m)
+        (1 - torch.nan_to_num(P, nan=1.0)).sum().backward()
+
+    def clock(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    for _ in range(args.warmup):
+        step()
+    t_step = clock(step, args.steps)
+    with torch.no_grad():
+        t_fwd = clock(lambda: matrix_profile(T.detach(), m), args.steps)
+    t64 = T.detach().double().contiguous()
+    init = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
+    prof = torch.full((B, n - m + 1), init, dtype=torch.int64, device=dev)
+    params = C.make_params(m)
+
+    def loop():
+        prof.fill_(init)
+        for b in range(B):
+            C.join_device(t64[b].data_ptr(), n, 0, 0, params, prof[b].data_ptr())
+
+    t_loop = clock(loop, args.steps)
+    line = {
+        "metric": "batched self-join pairs/sec (64 x n=8192, m=128): forward + backward",
+        "value": pairs / t_step, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_step * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 float32 random walks, n=8192 (PCG64 seeds 100..163), "
+                               "m=128, differentiable 1NN_INDEX self-join + backward",
+                   "batch": B, "n": n, "window": m, "pairs": pairs},
+        "forward_ms": t_fwd * 1e3,
+        "forward_pairs_per_s": pairs / t_fwd,
+        "per_series_loop_ms": t_loop * 1e3,
+    }
+    print(json.dumps(line))
+
+
 def calibrate(steps=3, nbytes=1 << 31):
     """A plain device copy of ``nbytes`` (elementwise kernel: read nbytes,
     write nbytes), run ``steps`` times, so that FETCH_SIZE / WRITE_SIZE
@@ -229,7 +290,8 @@ def main():
                     help="which forms to time (profiling runs time one form)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the copy kernel used to scale HBM counters")
-    ap.add_argument("--workload", choices=("selfjoin-n4194304-m1024", "selfjoin-n33554432-m4096"),
+    ap.add_argument("--workload", choices=("selfjoin-n4194304-m1024", "selfjoin-n33554432-m4096",
+                                           "batch-64x8192-m128"),
                     default="selfjoin-n4194304-m1024")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -241,6 +303,9 @@ def main():
         return
     if args.workload == "selfjoin-n33554432-m4096":
         large_selfjoin(args)
+        return
+    if args.workload == "batch-64x8192-m128":
+        batch_selfjoin(args)
         return
 
     import torch

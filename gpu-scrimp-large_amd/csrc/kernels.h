@@ -105,6 +105,11 @@ struct SweepArgs {
   int32_t knn_k;
   double thr;
   unsigned int *counter; // work counter, zeroed before launch
+  // Batched self-joins: nbatch series of one length stored back to back,
+  // every per-subsequence array with stride batch_stride entries between
+  // series; work item b is band b % nbands of series b / nbands; seeds are
+  // 2 * nbands * 512 entries per series.  nbatch = 1: a single join.
+  int64_t nbatch, batch_stride;
 };
 
 // cols/rows: which profiles the pass updates (for kMatrix: direct cells

@@ -156,9 +156,13 @@ struct Pass {
 void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
               bool fp32, SweepKind kind, uint32_t *mat, int64_t mat_w,
               int64_t rpc, int64_t cpc, double thr, int64_t anchor, int knn_k,
-              float *sweep_ms) {
+              float *sweep_ms, int64_t nbatch = 1, int64_t stride = 0,
+              int64_t ns_each = 0) {
+  // nbatch > 1: nbatch self-joins of series of ns_each subsequences stored
+  // with `stride` entries between series (p.X == p.Y holds all of them)
   hipStream_t s = ctx->stream;
-  const int64_t nX = p.X->ns, nY = p.Y->ns;
+  const int64_t nX = nbatch > 1 ? ns_each : p.X->ns;
+  const int64_t nY = nbatch > 1 ? ns_each : p.Y->ns;
   const int64_t ndiag = p.d_hi - p.d_lo;
   if (ndiag <= 0 || (!p.cols && !p.rows)) return;
   // Bands start on multiples of kDiagsPerLane (the sweep indexes its column
@@ -168,34 +172,43 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
 
   // First row of each band (a multiple of 8 below the band's full-row
   // limit) and the seed tasks.
-  std::vector<int64_t> r0(nbands);
+  std::vector<int64_t> r0(nbands * nbatch);
   std::vector<SeedTask> tasks;
-  for (int64_t b = 0; b < nbands; ++b) {
-    const int64_t D0 = d_base + b * kBand;
-    const int64_t dfirst = std::max(D0, p.d_lo);
-    const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
-    const int64_t rfull = std::min(nY, nX - dlast);
-    const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
-    r0[b] = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
-                      (uint64_t)slots) * kDiagsPerLane;
+  // per series: seedB (row r0 of each band) then seed0 (row 0 of every
+  // diagonal, indexed by diagonal - d_base); row offsets `series * stride`
+  const int64_t seeds_per_series = 2 * nbands * kBand;
+  for (int64_t q = 0; q < nbatch; ++q) {
+    const int64_t roff = q * stride, soff = q * seeds_per_series;
+    for (int64_t b = 0; b < nbands; ++b) {
+      const int64_t D0 = d_base + b * kBand;
+      const int64_t dfirst = std::max(D0, p.d_lo);
+      const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
+      const int64_t rfull = std::min(nY, nX - dlast);
+      const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
+      const int64_t rb = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
+                                   (uint64_t)slots) * kDiagsPerLane;
+      r0[q * nbands + b] = rb;
+      if (!fp32) {
+        const int64_t cnt = dlast - dfirst + 1;
+        for (int64_t j = 0; j < cnt; j += 256)
+          tasks.push_back(SeedTask{roff + rb, dfirst + j,
+                                   soff + b * kBand + (dfirst - D0) + j,
+                                   (int32_t)std::min<int64_t>(256, cnt - j), 0});
+      }
+    }
     if (!fp32) {
-      const int64_t cnt = dlast - dfirst + 1;
-      for (int64_t j = 0; j < cnt; j += 256)
-        tasks.push_back(SeedTask{r0[b], dfirst + j, b * kBand + (dfirst - D0) + j,
-                                 (int32_t)std::min<int64_t>(256, cnt - j), 0});
+      for (int64_t d = 0; d < ndiag; d += 256)
+        tasks.push_back(SeedTask{roff, p.d_lo + d,
+                                 soff + nbands * kBand + (p.d_lo - d_base) + d,
+                                 (int32_t)std::min<int64_t>(256, ndiag - d), 0});
     }
   }
-  int64_t *d_r0 = ar.alloc<int64_t>(nbands);
-  HIP_CHECK(hipMemcpyAsync(d_r0, r0.data(), sizeof(int64_t) * nbands,
+  int64_t *d_r0 = ar.alloc<int64_t>(nbands * nbatch);
+  HIP_CHECK(hipMemcpyAsync(d_r0, r0.data(), sizeof(int64_t) * nbands * nbatch,
                            hipMemcpyHostToDevice, s));
   double *seedB = nullptr, *seed0 = nullptr;
   if (!fp32) {
-    // seed0 (row 0 of every diagonal, indexed by diagonal - d_base) after
-    // seedB (row r0 of each band)
-    for (int64_t d = 0; d < ndiag; d += 256)
-      tasks.push_back(SeedTask{0, p.d_lo + d, nbands * kBand + (p.d_lo - d_base) + d,
-                               (int32_t)std::min<int64_t>(256, ndiag - d), 0});
-    double *seeds = ar.alloc<double>(2 * nbands * kBand);
+    double *seeds = ar.alloc<double>(seeds_per_series * nbatch);
     SeedTask *d_tasks = ar.alloc<SeedTask>((int64_t)tasks.size());
     HIP_CHECK(hipMemcpyAsync(d_tasks, tasks.data(),
                              sizeof(SeedTask) * tasks.size(),
@@ -226,8 +239,11 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
   a.mat_mirror = p.X == p.Y && rpc == cpc;
   a.counter = counter;
+  a.nbatch = nbatch;
+  a.batch_stride = stride;
 
-  const int grid = (int)std::min<int64_t>(nbands, (int64_t)ctx->num_cus * 16);
+  const int grid =
+      (int)std::min<int64_t>(nbands * nbatch, (int64_t)ctx->num_cus * 16);
   Event e0, e1;
   HIP_CHECK(hipEventRecord(e0.e, s));
   HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s));
@@ -450,6 +466,98 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   return SCAMP_HIP_OK;
 }
 
+// `batch` independent self-joins of series of length n stored back to back
+// in d_T, in one persistent sweep whose work items are (series, band).  Each
+// series gets its own padded segment of `seg` entries in every per-series
+// array (cleaned samples, statistics, profile keys), so the sweep's reads and
+// writes past a series' last subsequence land in that series' padding.
+int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
+                   int64_t batch, const scamp_hip_params *p, void *d_prof) {
+  const int64_t m = p->window;
+  const int pt = p->profile_type;
+  const int prec = p->precision;
+  const bool fp32 = prec == SCAMP_HIP_MIXED || prec == SCAMP_HIP_SINGLE;
+  int64_t anchor = p->anchor_interval > 0 ? p->anchor_interval : 8192;
+  anchor = std::max<int64_t>(16, (anchor + 15) / 16 * 16);
+  const int64_t ns = n - m + 1;
+  const int64_t seg = (n + kColumnPad + 63) / 64 * 64;
+  hipStream_t s = ctx->stream;
+  DevArena ar;
+  Event ev_start, ev_end;
+  HIP_CHECK(hipEventRecord(ev_start.e, s));
+
+  SeriesStats A;
+  A.n = n;
+  A.ns = ns;
+  A.clean = ar.alloc<double>(batch * seg);
+  A.mean = ar.alloc<double>(batch * seg);
+  A.inorm = ar.alloc<double>(batch * seg);
+  A.df = ar.alloc<double>(batch * seg);
+  A.dg = ar.alloc<double>(batch * seg);
+  for (int64_t q = 0; q < batch; ++q)
+    HIP_CHECK(launch_stats(d_T + q * n, n, m, A.clean + q * seg, seg,
+                           A.mean + q * seg, A.inorm + q * seg, A.df + q * seg,
+                           A.dg + q * seg, seg, s));
+  if (fp32) {
+    A.f_inorm = ar.alloc<float>(batch * seg);
+    A.f_df = ar.alloc<float>(batch * seg);
+    A.f_dg = ar.alloc<float>(batch * seg);
+    HIP_CHECK(launch_to_f32(A.inorm, A.f_inorm, batch * seg, s));
+    HIP_CHECK(launch_to_f32(A.df, A.f_df, batch * seg, s));
+    HIP_CHECK(launch_to_f32(A.dg, A.f_dg, batch * seg, s));
+  }
+
+  // keys[q * seg + j]: subsequence j of series q; the prior profiles are
+  // copied in row by row (pitch seg) and the padding holds "no match"
+  uint64_t *key = ar.alloc<uint64_t>(batch * seg);
+  const uint64_t init = make_key(-2.0f, 0xFFFFFFFFu);
+  HIP_CHECK(launch_fill_u64(key, batch * seg, init, s));
+  if (pt == SCAMP_HIP_1NN_INDEX) {
+    HIP_CHECK(hipMemcpy2DAsync(key, sizeof(uint64_t) * seg, d_prof,
+                               sizeof(uint64_t) * ns, sizeof(uint64_t) * ns,
+                               batch, hipMemcpyDeviceToDevice, s));
+    for (int64_t q = 0; q < batch; ++q)
+      HIP_CHECK(launch_mp_to_keys(key + q * seg, ns, s));
+  } else {
+    for (int64_t q = 0; q < batch; ++q)
+      HIP_CHECK(launch_f32_to_keys(static_cast<const float *>(d_prof) + q * ns,
+                                   key + q * seg, ns, s));
+  }
+
+  // diagonals [m/4, ns) of each series, rows and columns into one profile
+  // (src/tile.cpp:12-38 keeps |i - j| < m/4 out)
+  float sweep_ms = 0;
+  for (auto &r : cut(m / 4, ns, 0, 0)) {
+    Pass ps{&A, &A, r.first, r.second, true, true, key, key,
+            nullptr, nullptr, nullptr, nullptr};
+    run_pass(ctx, ar, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor, 0,
+             &sweep_ms, batch, seg, ns);
+  }
+
+  if (pt == SCAMP_HIP_1NN_INDEX) {
+    HIP_CHECK(launch_keys_to_mp(key, batch * seg, s));
+    HIP_CHECK(hipMemcpy2DAsync(d_prof, sizeof(uint64_t) * ns, key,
+                               sizeof(uint64_t) * seg, sizeof(uint64_t) * ns,
+                               batch, hipMemcpyDeviceToDevice, s));
+  } else {
+    for (int64_t q = 0; q < batch; ++q)
+      HIP_CHECK(launch_keys_to_f32(key + q * seg,
+                                   static_cast<float *>(d_prof) + q * ns, ns, s));
+  }
+  HIP_CHECK(hipEventRecord(ev_end.e, s));
+  HIP_CHECK(hipEventSynchronize(ev_end.e));
+  float total = 0;
+  HIP_CHECK(hipEventElapsedTime(&total, ev_start.e, ev_end.e));
+  ctx->total_ms = total;
+  ctx->sweep_ms = sweep_ms;
+  ctx->prep_ms = total - sweep_ms;
+  if (!p->silent_mode)
+    std::printf("scamp_hip: %lld self-joins of %lld subsequences in %.3f ms "
+                "(sweep %.3f ms)\n",
+                (long long)batch, (long long)ns, total, sweep_ms);
+  return SCAMP_HIP_OK;
+}
+
 size_t profile_elem_bytes(int pt) {
   switch (pt) {
     case SCAMP_HIP_1NN_INDEX: return sizeof(uint64_t);
@@ -527,6 +635,39 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
     HIP_CHECK(hipSetDevice(ctx->device));
     return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, p, d_profile_a,
                        d_profile_b);
+  } catch (const HipFailure &f) {
+    (void)hipStreamSynchronize(ctx->stream);
+    int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
+                                            : SCAMP_HIP_DEVICE_ERROR;
+    return fail(code, std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+}
+
+int scamp_hip_selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T,
+                              size_t n, int64_t batch,
+                              const scamp_hip_params *p, void *d_profiles) {
+  if (!ctx || !p || !d_T || !d_profiles)
+    return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  if (batch < 1) return fail(SCAMP_HIP_INVALID_ARGS, "batch must be >= 1");
+  if (p->has_b)
+    return fail(SCAMP_HIP_INVALID_ARGS, "a batched join is a self-join");
+  if (p->profile_type != SCAMP_HIP_1NN_INDEX &&
+      p->profile_type != SCAMP_HIP_1NN)
+    return fail(SCAMP_HIP_UNIMPLEMENTED,
+                "batched joins support 1NN_INDEX and 1NN profiles");
+  if (p->precision != SCAMP_HIP_SINGLE && p->precision != SCAMP_HIP_MIXED &&
+      p->precision != SCAMP_HIP_DOUBLE && p->precision != SCAMP_HIP_ULTRA)
+    return fail(SCAMP_HIP_INVALID_ARGS, "invalid precision type");
+  if (const char *msg = validate(p, (int64_t)n, 0))
+    return fail(SCAMP_HIP_INVALID_ARGS, msg);
+  // (series, band) work items are counted in 32 bits by the sweep
+  const int64_t bands = ((int64_t)n - p->window + 1) / kBand + 2;
+  if (batch > (int64_t)(0x7FFFFFFF / bands))
+    return fail(SCAMP_HIP_INVALID_ARGS, "batch too large for one launch");
+  try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_CHECK(hipSetDevice(ctx->device));
+    return selfjoin_batch(ctx, d_T, (int64_t)n, batch, p, d_profiles);
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
     int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY

@@ -796,18 +796,42 @@ void sweep_kernel(SweepArgs a) {
   __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
   __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
   const int lane = threadIdx.x;
+  const unsigned int items = (unsigned int)(a.nbands * a.nbatch);
   for (;;) {
     unsigned int b = 0;
     if (lane == 0) b = atomicAdd(a.counter, 1u);
     b = (unsigned int)__builtin_amdgcn_readfirstlane((int)b);
-    if (b >= a.nbands) break;
-    const int64_t D0 = a.d_base + (int64_t)b * kBand;
-    Sweep<DT, KIND, COLS, ROWS> sw(a, lane, D0);
-    const int64_t d_first = D0 > a.d_lo ? D0 : a.d_lo;
-    const int64_t dlast = D0 + kBand < a.d_hi ? D0 + kBand - 1 : a.d_hi - 1;
-    const int64_t rows0 = a.nY < a.nX - d_first ? a.nY : a.nX - d_first;
-    const int64_t rfull0 = a.nY < a.nX - dlast ? a.nY : a.nX - dlast;
-    const int64_t r0 = a.band_r0[b];
+    if (b >= items) break;
+    // batched joins: this work item's series, seen through offset pointers
+    const int64_t series = (int64_t)(b / (unsigned int)a.nbands);
+    const int64_t bl = (int64_t)b - series * a.nbands;
+    SweepArgs la = a;
+    if (a.nbatch > 1) {
+      const int64_t o = series * a.batch_stride;
+      la.dfX = static_cast<const DT *>(a.dfX) + o;
+      la.dgX = static_cast<const DT *>(a.dgX) + o;
+      la.inX = static_cast<const DT *>(a.inX) + o;
+      la.dfY = static_cast<const DT *>(a.dfY) + o;
+      la.dgY = static_cast<const DT *>(a.dgY) + o;
+      la.inY = static_cast<const DT *>(a.inY) + o;
+      la.TX = a.TX + o;
+      la.TY = a.TY + o;
+      la.muX = a.muX + o;
+      la.muY = a.muY + o;
+      la.keyX = a.keyX + o;
+      la.keyY = a.keyY + o;
+      if (sizeof(DT) == 8) {
+        la.seedB = a.seedB + series * 2 * a.nbands * kBand;
+        la.seed0 = la.seedB + a.nbands * kBand;
+      }
+    }
+    const int64_t D0 = la.d_base + bl * kBand;
+    Sweep<DT, KIND, COLS, ROWS> sw(la, lane, D0);
+    const int64_t d_first = D0 > la.d_lo ? D0 : la.d_lo;
+    const int64_t dlast = D0 + kBand < la.d_hi ? D0 + kBand - 1 : la.d_hi - 1;
+    const int64_t rows0 = la.nY < la.nX - d_first ? la.nY : la.nX - d_first;
+    const int64_t rfull0 = la.nY < la.nX - dlast ? la.nY : la.nX - dlast;
+    const int64_t r0 = la.band_r0[b];
     // segment 0: rows [r0, rows0) from the band's own seed at row r0;
     // segment 1: rows [0, r0) from the row-0 seeds (all diagonals valid).
     for (int seg = 0; seg < 2; ++seg) {
@@ -816,8 +840,8 @@ void sweep_kernel(SweepArgs a) {
       const int64_t rfull = seg == 0 ? rfull0 : r0;
       if (rs >= re) continue;
       if (sizeof(DT) == 8) {
-        sw.seed_from(seg == 0 ? a.seedB + (int64_t)b * kBand + (int64_t)lane * K
-                              : a.seed0 + (sw.d0 - a.d_base));
+        sw.seed_from(seg == 0 ? la.seedB + bl * kBand + (int64_t)lane * K
+                              : la.seed0 + (sw.d0 - la.d_base));
       } else {
         sw.anchor(rs, s_ax, s_ay);
       }

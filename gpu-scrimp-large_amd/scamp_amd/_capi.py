@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "FUNCTIONALITY_UNIMPLEMENTED", 2: "ILLEGAL_TYPE",
 EXPORTS = ("scamp_hip_ctx_create", "scamp_hip_ctx_destroy", "scamp_hip_device_count",
            "scamp_hip_last_error", "scamp_hip_join", "scamp_hip_join_device",
            "scamp_hip_stats", "scamp_hip_qt_seed", "scamp_hip_last_timing",
-           "scamp_hip_pearson_backward")
+           "scamp_hip_pearson_backward", "scamp_hip_selfjoin_batch")
 
 
 class Params(ctypes.Structure):
@@ -82,8 +82,8 @@ def load():
     lib.scamp_hip_stats.argtypes = [vp, vp, sz, i64, vp, vp, vp, vp]
     lib.scamp_hip_qt_seed.argtypes = [vp, vp, sz, vp, sz, i64, i64, i64, i64, vp]
     lib.scamp_hip_last_timing.argtypes = [vp, dp, dp, dp]
-    if hasattr(lib, "scamp_hip_pearson_backward"):
-        lib.scamp_hip_pearson_backward.argtypes = [vp, vp, sz, i64, i64, vp, vp, vp]
+    lib.scamp_hip_pearson_backward.argtypes = [vp, vp, sz, i64, i64, vp, vp, vp]
+    lib.scamp_hip_selfjoin_batch.argtypes = [vp, vp, sz, i64, ctypes.POINTER(Params), vp]
     for name in EXPORTS:
         if hasattr(lib, name):
             getattr(lib, name).restype = ctypes.c_int
@@ -257,3 +257,14 @@ def join_device(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=None, devic
         ctx.handle, ctypes.c_void_p(ptr_a), na, ctypes.c_void_p(ptr_b) if ptr_b else None,
         nb, ctypes.byref(params), ctypes.c_void_p(ptr_pa),
         ctypes.c_void_p(ptr_pb) if ptr_pb else None))
+
+
+def selfjoin_batch_device(ptr_T, n, batch, params: Params, ptr_profiles, device=0):
+    """`batch` self-joins of the float64 device series at ptr_T ([batch, n],
+    back to back) in one launch (scamp_hip_selfjoin_batch); the profiles at
+    ptr_profiles ([batch, n - m + 1], 1NN_INDEX or 1NN) are the prior and
+    are updated in place.  Blocks until done."""
+    ctx = context(device)
+    check(ctx._lib.scamp_hip_selfjoin_batch(
+        ctx.handle, ctypes.c_void_p(ptr_T), n, batch, ctypes.byref(params),
+        ctypes.c_void_p(ptr_profiles)))

@@ -5,8 +5,10 @@
 profile as Pearson correlations ``P`` [B, n-m+1] (same dtype as T; NaN where
 a window has no match) and the indices ``I`` [B, n-m+1] (int32, -1 = none).
 
-Forward: the plain FP64 join of each series (statistics, exact seeds,
-diagonal sweep) through the C ABI's device entry point.  Backward: the HIP
+Forward: the plain FP64 self-join of every series (statistics, exact seeds,
+diagonal sweep) in one batched launch through the C ABI, whose persistent
+sweep takes (series, band) work items, so a batch of short series fills the
+GPU as one long series does.  Backward: the HIP
 kernel ``scamp_hip_pearson_backward``, which reuses the forward's indices
 (no join is recomputed) and scatters dP_i/dT into windows i and I_i:
     dr/dx = (y^ - r x^) / |x~|,   dr/dy = (x^ - r y^) / |y~|.
@@ -25,12 +27,12 @@ _INIT = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
 _LOW = 0xFFFFFFFF
 
 
-def _forward_one(t64: torch.Tensor, m: int, device: int):
-    n = t64.numel()
-    prof = torch.full((n - m + 1,), _INIT, dtype=torch.int64, device=t64.device)
+def _forward(t64: torch.Tensor, m: int, device: int):
+    B, n = t64.shape
+    prof = torch.full((B, n - m + 1), _INIT, dtype=torch.int64, device=t64.device)
     torch.cuda.synchronize(t64.device)
-    C.join_device(t64.data_ptr(), n, 0, 0, C.make_params(m, precision="double"),
-                  prof.data_ptr(), device=device)
+    C.selfjoin_batch_device(t64.data_ptr(), n, B, C.make_params(m, precision="double"),
+                            prof.data_ptr(), device=device)
     corr = (prof & _LOW).to(torch.int32).view(torch.float32)
     idx = (prof >> 32).to(torch.int32)
     return corr, idx
@@ -46,11 +48,7 @@ class MatrixProfile(torch.autograd.Function):
             T = T.unsqueeze(0)
         device = T.device.index or 0
         t64 = T.detach().to(torch.float64).contiguous()
-        B, n = t64.shape
-        P = torch.empty((B, n - m + 1), dtype=torch.float32, device=T.device)
-        I = torch.empty((B, n - m + 1), dtype=torch.int32, device=T.device)
-        for b in range(B):
-            P[b], I[b] = _forward_one(t64[b], m, device)
+        P, I = _forward(t64, m, device)
         I = torch.where(P < -1, torch.full_like(I, -1), I)
         P = torch.where(P < -1, torch.full_like(P, float("nan")), P)
         ctx.save_for_backward(t64, I)

@@ -128,6 +128,18 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
                           const scamp_hip_params *p, void *d_profile_a,
                           void *d_profile_b);
 
+/* Many self-joins of equal-length series in one launch (no counterpart in
+ * the reference, whose callers loop over do_SCAMP per series,
+ * python/SCAMP_python.cpp:67-110).  d_T holds `batch` float64 series of n
+ * samples back to back (DEVICE); d_profiles holds batch * (n - m + 1)
+ * entries of a 1NN_INDEX (uint64 mp_entry) or 1NN (float) profile, read as
+ * the prior best-so-far and updated in place.  Equals `batch` calls of
+ * scamp_hip_join_device with has_b = 0; fills the GPU when each series alone
+ * has too few diagonals to.  Blocks until done. */
+int scamp_hip_selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, size_t n,
+                             int64_t batch, const scamp_hip_params *p,
+                             void *d_profiles);
+
 /* Statistics of one series on the GPU (replaces compute_statistics_cpu,
  * src/cpu_stats.cpp:123-160, with direct compensated per-window sums).
  * Host buffers: T[n]; outputs means/inorms/df/dg have n - m + 1 entries. */

@@ -1,0 +1,105 @@
+"""The batched self-join (scamp_hip_selfjoin_batch): `batch` series in one
+launch must give exactly what one scamp_hip_join_device self-join per series
+gives (same statistics, seeds, band rows and merges, so bit-identical
+profiles), for FP64 and the mixed form, 1NN_INDEX and 1NN, with NaN runs,
+with prior profiles that must be kept where they are better, and for series
+shorter than one 512-diagonal band as well as several bands long.  The
+per-series join is itself checked against the oracle by the parity tests."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import random_walk
+from scamp_amd import _capi as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _per_series(X, m, profile_type, precision, prior):
+    B, n = X.shape
+    out = prior.clone()
+    T = torch.tensor(X, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    params = C.make_params(m, profile_type, precision=precision)
+    for b in range(B):
+        C.join_device(T[b].data_ptr(), n, 0, 0, params, out[b].data_ptr())
+    return out
+
+
+def _batched(X, m, profile_type, precision, prior):
+    B, n = X.shape
+    out = prior.clone()
+    T = torch.tensor(X, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    C.selfjoin_batch_device(T.data_ptr(), n, B,
+                            C.make_params(m, profile_type, precision=precision),
+                            out.data_ptr())
+    return out
+
+
+def _prior(profile_type, B, ns, X, m, seed):
+    """Empty profiles, with a few seeded entries better than any real match
+    (corr 1, a marker index) that the join must keep."""
+    init = C.init_profile(profile_type, ns)
+    P = np.tile(init, (B, 1))
+    rng = np.random.default_rng(seed)
+    for b in range(B):
+        cols = rng.choice(ns, 5, replace=False)
+        if profile_type == "1NN_INDEX":
+            e = np.zeros(1, np.uint64)
+            e.view(np.float32)[0] = 1.0
+            e.view(np.uint32)[1] = 7
+            P[b, cols] = e[0]
+        else:
+            P[b, cols] = 1.0
+    t = torch.tensor(P.view(np.int64) if P.dtype == np.uint64 else P, device="cuda")
+    return t
+
+
+@pytest.mark.parametrize("B,n,m,precision,profile_type", [
+    (3, 700, 32, "double", "1NN_INDEX"),      # one band per series
+    (17, 2500, 64, "double", "1NN_INDEX"),    # several bands
+    (5, 5000, 100, "mixed", "1NN_INDEX"),
+    (4, 3000, 50, "double", "1NN"),
+    (6, 1800, 40, "mixed", "1NN"),
+    (1, 4000, 128, "double", "1NN_INDEX"),
+])
+def test_batch_equals_per_series_joins(B, n, m, precision, profile_type):
+    X = np.stack([random_walk(n, 500 + 13 * b, nan_runs=b % 3) for b in range(B)])
+    ns = n - m + 1
+    prior = _prior(profile_type, B, ns, X, m, B * n)
+    want = _per_series(X, m, profile_type, precision, prior)
+    got = _batched(X, m, profile_type, precision, prior)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    # the seeded prior entries survive (nothing beats corr 1 at index 7)
+    empty = torch.tensor(C.init_profile(profile_type, 1)[:1].view(
+        np.int64 if profile_type == "1NN_INDEX" else np.float32), device="cuda")
+    seeded = prior != empty
+    assert int(seeded.sum()) == 5 * B
+    assert torch.equal(got[seeded], prior[seeded])
+
+
+def test_batch_argument_errors():
+    lib = C.load()
+    ctx = C.context(0)
+    T = torch.zeros(2, 1000, dtype=torch.float64, device="cuda")
+    out = torch.zeros(2, 1000, dtype=torch.int64, device="cuda")
+    p = C.make_params(50)
+    rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 0,
+                                      ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
+    assert rc == 16 and "batch" in C.last_error()
+    p = C.make_params(50, "SUM_THRESH")
+    rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 2,
+                                      ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
+    assert rc == 1
+    p = C.make_params(50, has_b=True)
+    rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 2,
+                                      ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
+    assert rc == 16 and "self-join" in C.last_error()
+    p = C.make_params(2000)
+    rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 2,
+                                      ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
+    assert rc == 16
