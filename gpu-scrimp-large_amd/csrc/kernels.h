@@ -50,6 +50,15 @@ hipError_t launch_keys_to_f32(const uint64_t *src, float *dst, int64_t n,
 hipError_t launch_f32_to_enc(float *p, int64_t n, hipStream_t s);
 hipError_t launch_enc_to_f32(float *p, int64_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+// Self-join 1NN warm start: each subsequence's entries take the best of its
+// correlations with the 8 windows just past the exclusion zone e on either
+// side (forward partners into keyRow, backward ones into keyCol; either may
+// be null; equal pointers merge both).  batch series, stride entries apart.
+hipError_t launch_warm_profile(const double *clean, const double *mean,
+                               const double *inorm, int64_t ns, int64_t m,
+                               int64_t e, int64_t batch, int64_t stride,
+                               uint64_t *keyCol, uint64_t *keyRow,
+                               hipStream_t s);
 
 // ---- backward of the differentiable join (backward_kernel.hip) ------------
 // grad_t[b, :] += sum over profile entries i of grad_p[b, i] * d corr(i, I_i)
@@ -75,6 +84,16 @@ hipError_t launch_knn_to_matches(uint64_t *lists, int64_t n, int k, uint64_t ini
 // ---- the diagonal sweep (sweep_kernel.hip) --------------------------------
 enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3, kKnn = 4 };
 
+// One work item of the sweep: rows [rs, re) of one 512-diagonal band, then
+// rows [rs2, re2) of it (empty when re2 <= rs2).
+struct SweepItem {
+  int64_t rs, re, rs2, re2;  // rs, rs2 are multiples of 8
+  int64_t seed, seed2;       // FP64 form: offsets in `seeds` of the band's 512
+                             // exact covariances at rows rs and rs2
+  int32_t band;              // band b: diagonals d_base + 512 b + [0, 512)
+  int32_t series;            // batched joins: the series
+};
+
 struct SweepArgs {
   // column side X / row side Y update terms and inverse norms (FP64 or FP32,
   // padded by kColumnPad entries)
@@ -82,15 +101,14 @@ struct SweepArgs {
   const void *dfY, *dgY, *inY;
   // clean series and means (FP32 form: in-kernel FP64 re-anchoring)
   const double *TX, *TY, *muX, *muY;
-  // FP64 form: exact seeds at row 0 (indexed by diagonal - d_base) and at
-  // each band's first row (kBand entries per band)
-  const double *seed0, *seedB;
-  const int64_t *band_r0;  // first row of each band (multiple of 8)
+  // work items, taken in order through `counter`, and their FP64 seeds
+  const SweepItem *items;
+  int64_t nitems;
+  const double *seeds;
   int64_t nX, nY;        // subsequence counts of X and Y
   int64_t d_lo, d_hi;    // diagonal range of this pass
   int64_t d_base;        // d_lo rounded down to a multiple of 8: band b
                          // covers diagonals d_base + 512 b + [0, 512)
-  int64_t nbands;
   int64_t m;
   int64_t anchor;        // FP32 form: rows between re-anchors (multiple of 16)
   uint64_t *keyX, *keyY; // 1NN / 1NN_INDEX profiles (columns, rows)
@@ -105,10 +123,9 @@ struct SweepArgs {
   int32_t knn_k;
   double thr;
   unsigned int *counter; // work counter, zeroed before launch
-  // Batched self-joins: nbatch series of one length stored back to back,
-  // every per-subsequence array with stride batch_stride entries between
-  // series; work item b is band b % nbands of series b / nbands; seeds are
-  // 2 * nbands * 512 entries per series.  nbatch = 1: a single join.
+  // Batched self-joins: nbatch series of one length, every per-subsequence
+  // array with stride batch_stride entries between series (an item's
+  // `series` selects them).  nbatch = 1: a single join.
   int64_t nbatch, batch_stride;
 };
 

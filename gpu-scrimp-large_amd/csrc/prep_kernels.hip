@@ -278,6 +278,117 @@ inline unsigned blocks_for(int64_t n, int bs) {
   return static_cast<unsigned>((n + bs - 1) / bs);
 }
 
+
+// Warm start of a self-join's 1NN profile.  Thread i evaluates its window
+// against the kWarm windows just past the exclusion zone on each side,
+// i + d and i - d for d in [e, e + kWarm), and merges the best into its own
+// entries: forward partners (pairs where i is the row) into keyRow[i],
+// backward ones (i the column) into keyCol[i].  These are pairs of the join,
+// so the final profile is unchanged, but the sweep's thresholds start near
+// their final values instead of at "no match".  Covariances:
+//   sum_t x_t (T[j + t] - mu_i) - (mu_j - mu_i) sum_t x_t,  x_t = T[i + t] - mu_i
+// (both factors centred near the windows' level, so a large offset of the
+// series costs no digits); the correlation is formed in the sweep's order,
+// (cov * inorm_col) * inorm_row.  One series per blockIdx.y (stride entries
+// apart).
+constexpr int kWarm = 8;
+
+// sum_t (T[i + t] - mi) (T[j + t] - mi)
+__device__ inline double warm_dot(const double *T, int64_t i, int64_t j,
+                                  int64_t m, double mi) {
+  double acc = 0.0;
+  for (int64_t t = 0; t < m; ++t) acc = fma(T[i + t] - mi, T[j + t] - mi, acc);
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void warm_kernel(
+    const double *__restrict__ T, const double *__restrict__ mu,
+    const double *__restrict__ inorm, int64_t ns, int64_t m, int64_t e,
+    int64_t stride, uint64_t *keyCol, uint64_t *keyRow) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const int64_t o = (int64_t)blockIdx.y * stride;
+  T += o;
+  mu += o;
+  inorm += o;
+  const double ini = inorm[i];
+  if (!(ini == ini)) return;  // a NaN window has no matches
+  const double mi = mu[i];
+  // f[w]: partner i + e + w, b[w]: partner i - e - w
+  double f[kWarm], b[kWarm];
+#pragma unroll
+  for (int w = 0; w < kWarm; ++w) f[w] = b[w] = 0.0;
+  double sx = 0.0;
+  const bool fwd_all = keyRow && i + e + kWarm - 1 < ns;
+  const bool bwd_all = keyCol && i - e - (kWarm - 1) >= 0;
+  if (fwd_all && bwd_all) {
+    // every partner valid: register windows over the partners' samples,
+    // kWarm steps of t per stage (loads past the window end read the
+    // padded series and meet x = 0)
+    double fw[2 * kWarm - 1], bw[2 * kWarm - 1];
+#pragma unroll
+    for (int q = 0; q < kWarm - 1; ++q) {
+      fw[kWarm + q] = T[i + e + q] - mi;
+      bw[kWarm + q] = T[i - e - (kWarm - 1) + q] - mi;
+    }
+    for (int64_t t0 = 0; t0 < m; t0 += kWarm) {
+#pragma unroll
+      for (int q = 0; q < kWarm - 1; ++q) {
+        fw[q] = fw[kWarm + q];
+        bw[q] = bw[kWarm + q];
+      }
+#pragma unroll
+      for (int q = 0; q < kWarm; ++q) {
+        fw[kWarm - 1 + q] = T[i + e + t0 + kWarm - 1 + q] - mi;
+        bw[kWarm - 1 + q] = T[i - e + t0 + q] - mi;
+      }
+#pragma unroll
+      for (int u = 0; u < kWarm; ++u) {
+        const double x = t0 + u < m ? T[i + t0 + u] - mi : 0.0;
+        sx += x;
+#pragma unroll
+        for (int w = 0; w < kWarm; ++w) {
+          f[w] = fma(x, fw[u + w], f[w]);              // T[i+e+w+t] - mi
+          b[w] = fma(x, bw[u + kWarm - 1 - w], b[w]);  // T[i-e-w+t] - mi
+        }
+      }
+    }
+  } else {
+    for (int w = 0; w < kWarm; ++w) {
+      if (keyRow && i + e + w < ns) f[w] = warm_dot(T, i, i + e + w, m, mi);
+      if (keyCol && i - e - w >= 0) b[w] = warm_dot(T, i, i - e - w, m, mi);
+    }
+    for (int64_t t = 0; t < m; ++t) sx += T[i + t] - mi;
+  }
+  uint64_t best_row = 0, best_col = 0;
+#pragma unroll
+  for (int w = 0; w < kWarm; ++w) {
+    const int64_t jf = i + e + w, jb = i - e - w;
+    if (keyRow && jf < ns) {
+      const double c = ((f[w] - (mu[jf] - mi) * sx) * inorm[jf]) * ini;
+      if (c == c) {
+        const uint64_t k = make_key((float)c, (uint32_t)jf);
+        best_row = k > best_row ? k : best_row;
+      }
+    }
+    if (keyCol && jb >= 0) {
+      const double c = ((b[w] - (mu[jb] - mi) * sx) * ini) * inorm[jb];
+      if (c == c) {
+        const uint64_t k = make_key((float)c, (uint32_t)jb);
+        best_col = k > best_col ? k : best_col;
+      }
+    }
+  }
+  if (keyRow == keyCol) {
+    const uint64_t best = best_row > best_col ? best_row : best_col;
+    uint64_t *p = keyRow + o + i;
+    if (best > *p) *p = best;
+  } else {
+    if (keyRow && best_row > keyRow[o + i]) keyRow[o + i] = best_row;
+    if (keyCol && best_col > keyCol[o + i]) keyCol[o + i] = best_col;
+  }
+}
+
 }  // namespace
 
 hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
@@ -354,6 +465,20 @@ hipError_t launch_enc_to_f32(float *p, int64_t n, hipStream_t s) {
 }
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
   SCAMP_SIMPLE_LAUNCH(fill_u64_kernel, n, p, n, v);
+}
+
+
+hipError_t launch_warm_profile(const double *clean, const double *mean,
+                               const double *inorm, int64_t ns, int64_t m,
+                               int64_t e, int64_t batch, int64_t stride,
+                               uint64_t *keyCol, uint64_t *keyRow,
+                               hipStream_t s) {
+  if (ns <= 0 || batch <= 0 || (!keyCol && !keyRow)) return hipSuccess;
+  hipLaunchKernelGGL(warm_kernel,
+                     dim3(blocks_for(ns, 256), static_cast<unsigned>(batch)),
+                     dim3(256), 0, s, clean, mean, inorm, ns, m, e, stride,
+                     keyCol, keyRow);
+  return hipGetLastError();
 }
 
 }  // namespace scamp_amd

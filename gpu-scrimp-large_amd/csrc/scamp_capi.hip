@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -170,54 +171,104 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   const int64_t d_base = p.d_lo - p.d_lo % kDiagsPerLane;
   const int64_t nbands = (p.d_hi - d_base + kBand - 1) / kBand;
 
-  // First row of each band (a multiple of 8 below the band's full-row
-  // limit) and the seed tasks.
-  std::vector<int64_t> r0(nbands * nbatch);
-  std::vector<SeedTask> tasks;
-  // per series: seedB (row r0 of each band) then seed0 (row 0 of every
-  // diagonal, indexed by diagonal - d_base); row offsets `series * stride`
-  const int64_t seeds_per_series = 2 * nbands * kBand;
-  for (int64_t q = 0; q < nbatch; ++q) {
-    const int64_t roff = q * stride, soff = q * seeds_per_series;
-    for (int64_t b = 0; b < nbands; ++b) {
-      const int64_t D0 = d_base + b * kBand;
+  // Work items.  Band b spans rows [0, rows0) (rows with at least one valid
+  // diagonal); all of its diagonals are valid below rfull.  Its rows are cut
+  // at r0 + j * R: r0 is a hashed multiple of 8 below rfull, so that items
+  // running at the same time work on different rows' profile entries; R is
+  // unbounded when the pass has enough bands to fill the GPU, otherwise
+  // short enough that the items do, and never below max(512, 4m) rows so the
+  // seeds stay cheap next to the rows they start.
+  std::vector<int64_t> rows0(nbands), r0(nbands);
+  int64_t total_rows = 0;
+  for (int64_t b = 0; b < nbands; ++b) {
+    const int64_t D0 = d_base + b * kBand;
+    const int64_t dfirst = std::max(D0, p.d_lo);
+    const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
+    rows0[b] = std::min(nY, nX - dfirst);
+    const int64_t rfull = std::min(nY, nX - dlast);
+    const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
+    r0[b] = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
+                      (uint64_t)slots) * kDiagsPerLane;
+    total_rows += rows0[b];
+  }
+  const int64_t wave_slots = (int64_t)ctx->num_cus * 12;
+  int64_t R = std::numeric_limits<int64_t>::max();
+  if (nbands < 2 * wave_slots) {
+    const int64_t floor_rows = std::max<int64_t>(512, 4 * m);
+    R = std::max(floor_rows, (total_rows + 4 * wave_slots - 1) / (4 * wave_slots));
+    R = (R + 15) / 16 * 16;
+  }
+  // Items in band order (of every series), each band's from row r0 up and
+  // then from row 0: the low diagonals, where a series' best matches mostly
+  // lie, come first, so the profile thresholds that let most rows skip their
+  // atomic merges rise early; the triangle's short high bands form the tail.
+  // Unchunked, one item holds both of a band's segments, [r0, rows0) and
+  // then [0, r0), so concurrent bands stay at unrelated rows.
+  std::vector<SweepItem> items;
+  std::vector<int64_t> cuts;
+  const bool chunked = R != std::numeric_limits<int64_t>::max();
+  for (int64_t b = 0; b < nbands; ++b) {
+    if (!chunked) {
+      for (int64_t q = 0; q < nbatch; ++q)
+        items.push_back(SweepItem{r0[b], rows0[b], 0, r0[b], 0, 0, (int32_t)b,
+                                  (int32_t)q});
+      continue;
+    }
+    // boundaries 0 < r0 + j R < rows0 (ascending), rows0
+    cuts.assign(1, 0);
+    int64_t c = r0[b];
+    if (R < rows0[b])
+      while (c - R > 0) c -= R;
+    for (; c < rows0[b]; c = (R > rows0[b] - c) ? rows0[b] : c + R)
+      if (c > 0) cuts.push_back(c);
+    cuts.push_back(rows0[b]);
+    // r0 is 0 or a boundary: the items from it upwards, then those below it
+    const size_t n_it = cuts.size() - 1;
+    const size_t first = std::lower_bound(cuts.begin(), cuts.end(), r0[b]) - cuts.begin();
+    for (int64_t q = 0; q < nbatch; ++q)
+      for (size_t j = 0; j < n_it; ++j) {
+        const size_t k = (first + j) % n_it;
+        items.push_back(SweepItem{cuts[k], cuts[k + 1], 0, 0, 0, 0, (int32_t)b,
+                                  (int32_t)q});
+      }
+  }
+  const int64_t nitems = (int64_t)items.size();
+  if (nitems >= (int64_t)0xFFFFFFFFu)
+    throw HipFailure{hipErrorInvalidValue, "too many sweep work items"};
+  double *seeds = nullptr;
+  if (!fp32) {
+    // the exact covariance of the band's diagonals at the first row of every
+    // segment
+    std::vector<SeedTask> tasks;
+    int64_t nseeds = 0;
+    auto seed_tasks = [&](const SweepItem &it, int64_t row) {
+      const int64_t off = nseeds;
+      nseeds += kBand;
+      const int64_t D0 = d_base + it.band * kBand;
       const int64_t dfirst = std::max(D0, p.d_lo);
       const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
-      const int64_t rfull = std::min(nY, nX - dlast);
-      const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
-      const int64_t rb = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
-                                   (uint64_t)slots) * kDiagsPerLane;
-      r0[q * nbands + b] = rb;
-      if (!fp32) {
-        const int64_t cnt = dlast - dfirst + 1;
-        for (int64_t j = 0; j < cnt; j += 256)
-          tasks.push_back(SeedTask{roff + rb, dfirst + j,
-                                   soff + b * kBand + (dfirst - D0) + j,
-                                   (int32_t)std::min<int64_t>(256, cnt - j), 0});
-      }
+      const int64_t cnt = dlast - dfirst + 1;
+      const int64_t roff = (int64_t)it.series * stride;
+      for (int64_t j = 0; j < cnt; j += 256)
+        tasks.push_back(SeedTask{roff + row, dfirst + j, off + (dfirst - D0) + j,
+                                 (int32_t)std::min<int64_t>(256, cnt - j), 0});
+      return off;
+    };
+    for (SweepItem &it : items) {
+      it.seed = seed_tasks(it, it.rs);
+      if (it.re2 > it.rs2) it.seed2 = seed_tasks(it, it.rs2);
     }
-    if (!fp32) {
-      for (int64_t d = 0; d < ndiag; d += 256)
-        tasks.push_back(SeedTask{roff, p.d_lo + d,
-                                 soff + nbands * kBand + (p.d_lo - d_base) + d,
-                                 (int32_t)std::min<int64_t>(256, ndiag - d), 0});
-    }
-  }
-  int64_t *d_r0 = ar.alloc<int64_t>(nbands * nbatch);
-  HIP_CHECK(hipMemcpyAsync(d_r0, r0.data(), sizeof(int64_t) * nbands * nbatch,
-                           hipMemcpyHostToDevice, s));
-  double *seedB = nullptr, *seed0 = nullptr;
-  if (!fp32) {
-    double *seeds = ar.alloc<double>(seeds_per_series * nbatch);
+    seeds = ar.alloc<double>(nseeds);
     SeedTask *d_tasks = ar.alloc<SeedTask>((int64_t)tasks.size());
     HIP_CHECK(hipMemcpyAsync(d_tasks, tasks.data(),
                              sizeof(SeedTask) * tasks.size(),
                              hipMemcpyHostToDevice, s));
     HIP_CHECK(launch_seeds(d_tasks, (int64_t)tasks.size(), p.X->clean,
                            p.X->mean, p.Y->clean, p.Y->mean, m, seeds, s));
-    seedB = seeds;
-    seed0 = seeds + nbands * kBand;
   }
+  SweepItem *d_items = ar.alloc<SweepItem>(nitems);
+  HIP_CHECK(hipMemcpyAsync(d_items, items.data(), sizeof(SweepItem) * nitems,
+                           hipMemcpyHostToDevice, s));
 
   unsigned int *counter = ar.alloc<unsigned int>(1);
   HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
@@ -231,9 +282,9 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
     a.dfY = p.Y->df; a.dgY = p.Y->dg; a.inY = p.Y->inorm;
   }
   a.TX = p.X->clean; a.TY = p.Y->clean; a.muX = p.X->mean; a.muY = p.Y->mean;
-  a.seed0 = seed0; a.seedB = seedB; a.band_r0 = d_r0;
+  a.items = d_items; a.nitems = nitems; a.seeds = seeds;
   a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi; a.d_base = d_base;
-  a.nbands = nbands; a.m = m; a.anchor = anchor;
+  a.m = m; a.anchor = anchor;
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
   a.knnX = p.knnX; a.knnY = p.knnY; a.knn_k = knn_k;
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
@@ -242,8 +293,7 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   a.nbatch = nbatch;
   a.batch_stride = stride;
 
-  const int grid =
-      (int)std::min<int64_t>(nbands * nbatch, (int64_t)ctx->num_cus * 16);
+  const int grid = (int)std::min<int64_t>(nitems, (int64_t)ctx->num_cus * 16);
   Event e0, e1;
   HIP_CHECK(hipEventRecord(e0.e, s));
   HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s));
@@ -388,6 +438,14 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     kind = (cpc >= kDiagsPerLane && rpc >= kDiagsPerLane) ? kMatrix
                                                           : kMatrixSmall;
   }
+  // Self-join 1NN profiles start from each subsequence's best match among
+  // the windows just past the exclusion zone (pairs of the join, so the
+  // result is the same; the sweep's thresholds then start near their final
+  // values, which short series would otherwise spend much of their rows on).
+  if (self && kind == kNN && (cols_out || rows_out))
+    HIP_CHECK(launch_warm_profile(A.clean, A.mean, A.inorm, nsA, m, e, 1, 0,
+                                  cols_out ? keyA : nullptr,
+                                  rows_out ? (sep_rows ? keyB : keyA) : nullptr, s));
   std::vector<Pass> passes;
   if (self) {
     for (auto &r : cut(e, nsA, 0, 0)) {
@@ -525,7 +583,10 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   }
 
   // diagonals [m/4, ns) of each series, rows and columns into one profile
-  // (src/tile.cpp:12-38 keeps |i - j| < m/4 out)
+  // (src/tile.cpp:12-38 keeps |i - j| < m/4 out), after the same warm start
+  // as a single self-join
+  HIP_CHECK(launch_warm_profile(A.clean, A.mean, A.inorm, ns, m, m / 4, batch,
+                                seg, key, key, s));
   float sweep_ms = 0;
   for (auto &r : cut(m / 4, ns, 0, 0)) {
     Pass ps{&A, &A, r.first, r.second, true, true, key, key,

@@ -796,18 +796,20 @@ void sweep_kernel(SweepArgs a) {
   __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
   __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
   const int lane = threadIdx.x;
-  const unsigned int items = (unsigned int)(a.nbands * a.nbatch);
+  const unsigned int nitems = (unsigned int)a.nitems;
   for (;;) {
     unsigned int b = 0;
     if (lane == 0) b = atomicAdd(a.counter, 1u);
     b = (unsigned int)__builtin_amdgcn_readfirstlane((int)b);
-    if (b >= items) break;
-    // batched joins: this work item's series, seen through offset pointers
-    const int64_t series = (int64_t)(b / (unsigned int)a.nbands);
-    const int64_t bl = (int64_t)b - series * a.nbands;
+    if (b >= nitems) break;
+    // the item is read where each field is needed (not held in registers
+    // across the first segment)
+    const SweepItem *it = a.items + b;
+    const int32_t band = it->band;
+    // batched joins: the item's series, seen through offset pointers
     SweepArgs la = a;
     if (a.nbatch > 1) {
-      const int64_t o = series * a.batch_stride;
+      const int64_t o = (int64_t)it->series * a.batch_stride;
       la.dfX = static_cast<const DT *>(a.dfX) + o;
       la.dgX = static_cast<const DT *>(a.dgX) + o;
       la.inX = static_cast<const DT *>(a.inX) + o;
@@ -820,31 +822,20 @@ void sweep_kernel(SweepArgs a) {
       la.muY = a.muY + o;
       la.keyX = a.keyX + o;
       la.keyY = a.keyY + o;
-      if (sizeof(DT) == 8) {
-        la.seedB = a.seedB + series * 2 * a.nbands * kBand;
-        la.seed0 = la.seedB + a.nbands * kBand;
-      }
     }
-    const int64_t D0 = la.d_base + bl * kBand;
+    const int64_t D0 = la.d_base + (int64_t)band * kBand;
     Sweep<DT, KIND, COLS, ROWS> sw(la, lane, D0);
-    const int64_t d_first = D0 > la.d_lo ? D0 : la.d_lo;
-    const int64_t dlast = D0 + kBand < la.d_hi ? D0 + kBand - 1 : la.d_hi - 1;
-    const int64_t rows0 = la.nY < la.nX - d_first ? la.nY : la.nX - d_first;
-    const int64_t rfull0 = la.nY < la.nX - dlast ? la.nY : la.nX - dlast;
-    const int64_t r0 = la.band_r0[b];
-    // segment 0: rows [r0, rows0) from the band's own seed at row r0;
-    // segment 1: rows [0, r0) from the row-0 seeds (all diagonals valid).
     for (int seg = 0; seg < 2; ++seg) {
-      const int64_t rs = seg == 0 ? r0 : 0;
-      const int64_t re = seg == 0 ? rows0 : r0;
-      const int64_t rfull = seg == 0 ? rfull0 : r0;
+      const int64_t rs = seg == 0 ? it->rs : it->rs2;
+      const int64_t re = seg == 0 ? it->re : it->re2;
       if (rs >= re) continue;
-      if (sizeof(DT) == 8) {
-        sw.seed_from(seg == 0 ? la.seedB + bl * kBand + (int64_t)lane * K
-                              : la.seed0 + (sw.d0 - la.d_base));
-      } else {
+      const int64_t dlast = D0 + kBand < la.d_hi ? D0 + kBand - 1 : la.d_hi - 1;
+      // all of the band's diagonals are valid on rows below rfull
+      const int64_t rfull = la.nY < la.nX - dlast ? la.nY : la.nX - dlast;
+      if (sizeof(DT) == 8)
+        sw.seed_from(la.seeds + (seg == 0 ? it->seed : it->seed2) + (int64_t)lane * K);
+      else
         sw.anchor(rs, s_ax, s_ay);
-      }
       sw.segment(rs, re, rfull, s_row, s_col, s_ax, s_ay);
     }
   }
