@@ -10,11 +10,12 @@ namespace scamp_amd {
 // ---- statistics (prep_kernels.hip) ----------------------------------------
 // T[n] raw series -> clean[n] (non-finite -> 0); means/inorms/df/dg[n-m+1].
 // Output arrays may be longer (padding): the pad is filled with mean 0,
-// inorm NaN, df 0, dg 0, clean 0.
+// inorm NaN, df 0, dg 0, clean 0.  batch > 1: series q at T + q * t_stride,
+// its outputs at q * clean_len (clean) and q * stat_len (the others).
 hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
                         int64_t clean_len, double *means, double *inorms,
                         double *df, double *dg, int64_t stat_len,
-                        hipStream_t s);
+                        hipStream_t s, int64_t batch = 1, int64_t t_stride = 0);
 
 // FP64 -> FP32 copies of the per-window update terms and inverse norms.
 hipError_t launch_to_f32(const double *src, float *dst, int64_t len,
@@ -50,6 +51,13 @@ hipError_t launch_keys_to_f32(const uint64_t *src, float *dst, int64_t n,
 hipError_t launch_f32_to_enc(float *p, int64_t n, hipStream_t s);
 hipError_t launch_enc_to_f32(float *p, int64_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+// batch profiles of n entries, back to back, in SCAMP's layout (mp: uint64
+// mp_entry, else float) <-> keys with kstride entries between series
+hipError_t launch_profile_in(const void *src, bool mp, uint64_t *keys,
+                             int64_t n, int64_t batch, int64_t kstride,
+                             hipStream_t s);
+hipError_t launch_profile_out(const uint64_t *keys, int64_t kstride, void *dst,
+                              bool mp, int64_t n, int64_t batch, hipStream_t s);
 // Self-join 1NN warm start: each subsequence's entries take the best of its
 // correlations with the 8 windows just past the exclusion zone e on either
 // side (forward partners into keyRow, backward ones into keyCol; either may

@@ -30,10 +30,15 @@ constexpr double kFlatness = 1e-13;
 // gaps, which the FP32 recurrence could not absorb without losing digits.
 constexpr int kHoldSearch = 4096;
 
+// Statistics kernels: one series per blockIdx.y, the input `t_stride` and
+// the outputs `clean_len` / `stat_len` entries apart.
 __global__ void clean_kernel(const double *__restrict__ T, int64_t n,
-                             double *__restrict__ clean, int64_t clean_len) {
+                             double *__restrict__ clean, int64_t clean_len,
+                             int64_t t_stride) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= clean_len) return;
+  T += blockIdx.y * t_stride;
+  clean += blockIdx.y * clean_len;
   if (i >= n) {
     clean[i] = 0.0;
     return;
@@ -66,7 +71,11 @@ __global__ void clean_kernel(const double *__restrict__ T, int64_t n,
 __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
     const double *__restrict__ T, const double *__restrict__ clean, int64_t n,
     int64_t m, double *__restrict__ means, double *__restrict__ inorms,
-    int64_t ns, int64_t stat_len) {
+    int64_t ns, int64_t stat_len, int64_t t_stride, int64_t clean_len) {
+  T += blockIdx.y * t_stride;
+  clean += blockIdx.y * clean_len;
+  means += blockIdx.y * stat_len;
+  inorms += blockIdx.y * stat_len;
   __shared__ double tile[kStatBlock + kStatChunk];
   __shared__ bool flag[kStatBlock + kStatChunk];
   const int64_t w0 = blockIdx.x * (int64_t)kStatBlock;
@@ -127,9 +136,13 @@ __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
 __global__ void dfdg_kernel(const double *__restrict__ clean, int64_t m,
                             const double *__restrict__ means, int64_t ns,
                             double *__restrict__ df, double *__restrict__ dg,
-                            int64_t stat_len) {
+                            int64_t stat_len, int64_t clean_len) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= stat_len) return;
+  clean += blockIdx.y * clean_len;
+  means += blockIdx.y * stat_len;
+  df += blockIdx.y * stat_len;
+  dg += blockIdx.y * stat_len;
   if (i < ns - 1) {
     double a = clean[i], b = clean[i + m];
     df[i] = (b - a) / 2.0;
@@ -257,6 +270,37 @@ __global__ void f32_to_keys_kernel(const float *src, uint64_t *dst, int64_t n) {
 __global__ void keys_to_f32_kernel(const uint64_t *src, float *dst, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) dst[i] = key_corr(src[i]);
+}
+
+// Profile conversions between SCAMP's layouts (n entries per series, series
+// back to back) and padded key arrays (`kstride` entries per series); one
+// series per blockIdx.y.
+__global__ void profile_in_kernel(const void *src, bool mp, uint64_t *keys,
+                                  int64_t n, int64_t kstride) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t q = blockIdx.y;
+  uint64_t k;
+  if (mp) {
+    const uint64_t e = static_cast<const uint64_t *>(src)[q * n + i];
+    k = make_key(bits_f32(static_cast<uint32_t>(e)), static_cast<uint32_t>(e >> 32));
+  } else {
+    k = make_key(static_cast<const float *>(src)[q * n + i], 0xFFFFFFFFu);
+  }
+  keys[q * kstride + i] = k;
+}
+
+__global__ void profile_out_kernel(const uint64_t *keys, int64_t kstride,
+                                   void *dst, bool mp, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t q = blockIdx.y;
+  const uint64_t k = keys[q * kstride + i];
+  if (mp)
+    static_cast<uint64_t *>(dst)[q * n + i] =
+        (static_cast<uint64_t>(key_index(k)) << 32) | f32_bits(key_corr(k));
+  else
+    static_cast<float *>(dst)[q * n + i] = key_corr(k);
 }
 
 __global__ void f32_to_enc_kernel(float *p, int64_t n) {
@@ -394,15 +438,17 @@ __global__ __launch_bounds__(256) void warm_kernel(
 hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
                         int64_t clean_len, double *means, double *inorms,
                         double *df, double *dg, int64_t stat_len,
-                        hipStream_t s) {
+                        hipStream_t s, int64_t batch, int64_t t_stride) {
   const int64_t ns = n - m + 1;
-  hipLaunchKernelGGL(clean_kernel, dim3(blocks_for(clean_len, 256)), dim3(256),
-                     0, s, T, n, clean, clean_len);
+  const unsigned nb = static_cast<unsigned>(batch);
+  hipLaunchKernelGGL(clean_kernel, dim3(blocks_for(clean_len, 256), nb),
+                     dim3(256), 0, s, T, n, clean, clean_len, t_stride);
   hipLaunchKernelGGL(window_stats_kernel,
-                     dim3(blocks_for(stat_len, kStatBlock)), dim3(kStatBlock),
-                     0, s, T, clean, n, m, means, inorms, ns, stat_len);
-  hipLaunchKernelGGL(dfdg_kernel, dim3(blocks_for(stat_len, 256)), dim3(256),
-                     0, s, clean, m, means, ns, df, dg, stat_len);
+                     dim3(blocks_for(stat_len, kStatBlock), nb), dim3(kStatBlock),
+                     0, s, T, clean, n, m, means, inorms, ns, stat_len, t_stride,
+                     clean_len);
+  hipLaunchKernelGGL(dfdg_kernel, dim3(blocks_for(stat_len, 256), nb), dim3(256),
+                     0, s, clean, m, means, ns, df, dg, stat_len, clean_len);
   return hipGetLastError();
 }
 
@@ -456,6 +502,23 @@ hipError_t launch_f32_to_keys(const float *src, uint64_t *dst, int64_t n,
 hipError_t launch_keys_to_f32(const uint64_t *src, float *dst, int64_t n,
                               hipStream_t s) {
   SCAMP_SIMPLE_LAUNCH(keys_to_f32_kernel, n, src, dst, n);
+}
+hipError_t launch_profile_in(const void *src, bool mp, uint64_t *keys,
+                             int64_t n, int64_t batch, int64_t kstride,
+                             hipStream_t s) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(profile_in_kernel,
+                     dim3(blocks_for(n, 256), static_cast<unsigned>(batch)),
+                     dim3(256), 0, s, src, mp, keys, n, kstride);
+  return hipGetLastError();
+}
+hipError_t launch_profile_out(const uint64_t *keys, int64_t kstride, void *dst,
+                              bool mp, int64_t n, int64_t batch, hipStream_t s) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(profile_out_kernel,
+                     dim3(blocks_for(n, 256), static_cast<unsigned>(batch)),
+                     dim3(256), 0, s, keys, kstride, dst, mp, n);
+  return hipGetLastError();
 }
 hipError_t launch_f32_to_enc(float *p, int64_t n, hipStream_t s) {
   SCAMP_SIMPLE_LAUNCH(f32_to_enc_kernel, n, p, n);

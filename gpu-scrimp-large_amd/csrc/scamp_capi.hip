@@ -37,6 +37,14 @@ struct scamp_hip_ctx {
   int num_cus = 0;
   float total_ms = 0, prep_ms = 0, sweep_ms = 0;
   std::mutex mu;
+  // Device blocks kept from earlier calls (bytes, pointer), reused by later
+  // ones: a short join would otherwise spend a noticeable part of its time
+  // in hipMalloc / hipFree.  Only blocks up to kCacheBlock bytes, at most
+  // kCacheTotal bytes in all; freed by scamp_hip_ctx_destroy.
+  std::vector<std::pair<size_t, void *>> cache;
+  size_t cached = 0;
+  static constexpr size_t kCacheBlock = size_t(256) << 20;
+  static constexpr size_t kCacheTotal = size_t(2) << 30;
 };
 
 namespace {
@@ -70,19 +78,52 @@ struct Event {
   Event &operator=(const Event &) = delete;
 };
 
-// Device allocations owned by one join; freed on scope exit.
+// Device allocations owned by one call (made with the context locked);
+// returned to the context's cache, or freed, on scope exit.  Work that still
+// reads a returned block is queued on the context's stream, ahead of any
+// later call's use of it.
 struct DevArena {
-  std::vector<void *> ptrs;
+  scamp_hip_ctx *ctx;
+  std::vector<std::pair<size_t, void *>> ptrs;
+  explicit DevArena(scamp_hip_ctx *c) : ctx(c) {}
+  DevArena(const DevArena &) = delete;
+  DevArena &operator=(const DevArena &) = delete;
   template <typename T>
   T *alloc(int64_t count) {
+    // 256-byte granules, so that blocks are reused across nearby sizes
+    const size_t bytes =
+        (sizeof(T) * (size_t)std::max<int64_t>(count, 1) + 255) / 256 * 256;
     void *p = nullptr;
-    size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(count, 1);
-    HIP_CHECK(hipMalloc(&p, bytes));
-    ptrs.push_back(p);
+    // the smallest cached block that fits and wastes at most half of it
+    size_t best = ctx->cache.size();
+    for (size_t i = 0; i < ctx->cache.size(); ++i) {
+      const size_t b = ctx->cache[i].first;
+      if (b >= bytes && b <= 2 * bytes &&
+          (best == ctx->cache.size() || b < ctx->cache[best].first))
+        best = i;
+    }
+    size_t got = bytes;
+    if (best < ctx->cache.size()) {
+      got = ctx->cache[best].first;
+      p = ctx->cache[best].second;
+      ctx->cached -= got;
+      ctx->cache.erase(ctx->cache.begin() + best);
+    } else {
+      HIP_CHECK(hipMalloc(&p, bytes));
+    }
+    ptrs.emplace_back(got, p);
     return static_cast<T *>(p);
   }
   ~DevArena() {
-    for (void *p : ptrs) (void)hipFree(p);
+    for (auto &b : ptrs) {
+      if (b.first <= scamp_hip_ctx::kCacheBlock &&
+          ctx->cached + b.first <= scamp_hip_ctx::kCacheTotal) {
+        ctx->cache.push_back(b);
+        ctx->cached += b.first;
+      } else {
+        (void)hipFree(b.second);
+      }
+    }
   }
 };
 
@@ -354,7 +395,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   anchor = std::max<int64_t>(16, (anchor + 15) / 16 * 16);
 
   hipStream_t s = ctx->stream;
-  DevArena ar;
+  DevArena ar(ctx);
   Event ev_start, ev_prep, ev_end;
   HIP_CHECK(hipEventRecord(ev_start.e, s));
 
@@ -540,7 +581,7 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   const int64_t ns = n - m + 1;
   const int64_t seg = (n + kColumnPad + 63) / 64 * 64;
   hipStream_t s = ctx->stream;
-  DevArena ar;
+  DevArena ar(ctx);
   Event ev_start, ev_end;
   HIP_CHECK(hipEventRecord(ev_start.e, s));
 
@@ -552,10 +593,8 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   A.inorm = ar.alloc<double>(batch * seg);
   A.df = ar.alloc<double>(batch * seg);
   A.dg = ar.alloc<double>(batch * seg);
-  for (int64_t q = 0; q < batch; ++q)
-    HIP_CHECK(launch_stats(d_T + q * n, n, m, A.clean + q * seg, seg,
-                           A.mean + q * seg, A.inorm + q * seg, A.df + q * seg,
-                           A.dg + q * seg, seg, s));
+  HIP_CHECK(launch_stats(d_T, n, m, A.clean, seg, A.mean, A.inorm, A.df, A.dg,
+                         seg, s, batch, n));
   if (fp32) {
     A.f_inorm = ar.alloc<float>(batch * seg);
     A.f_df = ar.alloc<float>(batch * seg);
@@ -565,22 +604,12 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
     HIP_CHECK(launch_to_f32(A.dg, A.f_dg, batch * seg, s));
   }
 
-  // keys[q * seg + j]: subsequence j of series q; the prior profiles are
-  // copied in row by row (pitch seg) and the padding holds "no match"
+  // keys[q * seg + j]: subsequence j of series q, from the prior profiles;
+  // the padding holds "no match"
   uint64_t *key = ar.alloc<uint64_t>(batch * seg);
-  const uint64_t init = make_key(-2.0f, 0xFFFFFFFFu);
-  HIP_CHECK(launch_fill_u64(key, batch * seg, init, s));
-  if (pt == SCAMP_HIP_1NN_INDEX) {
-    HIP_CHECK(hipMemcpy2DAsync(key, sizeof(uint64_t) * seg, d_prof,
-                               sizeof(uint64_t) * ns, sizeof(uint64_t) * ns,
-                               batch, hipMemcpyDeviceToDevice, s));
-    for (int64_t q = 0; q < batch; ++q)
-      HIP_CHECK(launch_mp_to_keys(key + q * seg, ns, s));
-  } else {
-    for (int64_t q = 0; q < batch; ++q)
-      HIP_CHECK(launch_f32_to_keys(static_cast<const float *>(d_prof) + q * ns,
-                                   key + q * seg, ns, s));
-  }
+  HIP_CHECK(launch_fill_u64(key, batch * seg, make_key(-2.0f, 0xFFFFFFFFu), s));
+  const bool mp = pt == SCAMP_HIP_1NN_INDEX;
+  HIP_CHECK(launch_profile_in(d_prof, mp, key, ns, batch, seg, s));
 
   // diagonals [m/4, ns) of each series, rows and columns into one profile
   // (src/tile.cpp:12-38 keeps |i - j| < m/4 out), after the same warm start
@@ -595,16 +624,7 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
              &sweep_ms, batch, seg, ns);
   }
 
-  if (pt == SCAMP_HIP_1NN_INDEX) {
-    HIP_CHECK(launch_keys_to_mp(key, batch * seg, s));
-    HIP_CHECK(hipMemcpy2DAsync(d_prof, sizeof(uint64_t) * ns, key,
-                               sizeof(uint64_t) * seg, sizeof(uint64_t) * ns,
-                               batch, hipMemcpyDeviceToDevice, s));
-  } else {
-    for (int64_t q = 0; q < batch; ++q)
-      HIP_CHECK(launch_keys_to_f32(key + q * seg,
-                                   static_cast<float *>(d_prof) + q * ns, ns, s));
-  }
+  HIP_CHECK(launch_profile_out(key, seg, d_prof, mp, ns, batch, s));
   HIP_CHECK(hipEventRecord(ev_end.e, s));
   HIP_CHECK(hipEventSynchronize(ev_end.e));
   float total = 0;
@@ -666,6 +686,8 @@ int scamp_hip_ctx_create(int device, scamp_hip_ctx **out) {
 int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx) {
   if (!ctx) return SCAMP_HIP_OK;
   (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &b : ctx->cache) (void)hipFree(b.second);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SCAMP_HIP_OK;
@@ -759,7 +781,7 @@ int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    DevArena ar;
+    DevArena ar(ctx);
     hipStream_t s = ctx->stream;
     double *d_a = ar.alloc<double>((int64_t)na);
     HIP_CHECK(hipMemcpyAsync(d_a, a, sizeof(double) * na, hipMemcpyHostToDevice, s));
@@ -800,7 +822,7 @@ int scamp_hip_stats(scamp_hip_ctx *ctx, const double *T, size_t n,
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    DevArena ar;
+    DevArena ar(ctx);
     hipStream_t s = ctx->stream;
     double *d_T = ar.alloc<double>((int64_t)n);
     HIP_CHECK(hipMemcpyAsync(d_T, T, sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -831,7 +853,7 @@ int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    DevArena ar;
+    DevArena ar(ctx);
     hipStream_t s = ctx->stream;
     double *dX = ar.alloc<double>((int64_t)nx);
     double *dY = ar.alloc<double>((int64_t)ny);
@@ -871,19 +893,18 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    DevArena ar;
+    DevArena ar(ctx);
     hipStream_t s = ctx->stream;
     const int64_t ns = (int64_t)n - m + 1;
     const int64_t stat_len = ns + kColumnPad;
+    const int64_t clean_len = (int64_t)n + kColumnPad;
     double *mean = ar.alloc<double>(batch * stat_len);
     double *inorm = ar.alloc<double>(batch * stat_len);
-    double *clean = ar.alloc<double>((int64_t)n + kColumnPad);
-    double *df = ar.alloc<double>(stat_len);
-    double *dg = ar.alloc<double>(stat_len);
-    for (int64_t b = 0; b < batch; ++b)
-      HIP_CHECK(launch_stats(d_T + b * n, (int64_t)n, m, clean,
-                             (int64_t)n + kColumnPad, mean + b * stat_len,
-                             inorm + b * stat_len, df, dg, stat_len, s));
+    double *clean = ar.alloc<double>(batch * clean_len);
+    double *df = ar.alloc<double>(batch * stat_len);
+    double *dg = ar.alloc<double>(batch * stat_len);
+    HIP_CHECK(launch_stats(d_T, (int64_t)n, m, clean, clean_len, mean, inorm,
+                           df, dg, stat_len, s, batch, (int64_t)n));
     HIP_CHECK(hipMemsetAsync(d_grad_T, 0, sizeof(double) * n * batch, s));
     HIP_CHECK(launch_pearson_backward(d_T, (int64_t)n, m, batch, mean, inorm,
                                       stat_len, d_index, d_grad_corr, d_grad_T,
