@@ -162,32 +162,63 @@ __global__ void to_f32_kernel(const double *__restrict__ src,
 // Exact-summation seeds: Ogita-Rump-Oishi Dot2 (TwoProduct via FMA +
 // TwoSum), the summation the reference's MATLAB notes describe
 // (test/dot2s.m); accurate to about one rounding of the true sum of the
-// centred products.  Thread j of a task owns diagonal diag0 + j; the row
-// window Y is shared by the task (uniform loads), X is read coalesced.
+// centred products.  Thread j of a task owns diagonal diag0 + j.  The task's
+// X samples and centred Y window are staged through LDS, kSeedChunk values of
+// t at a time; each thread keeps four Dot2 accumulators over interleaved t
+// (independent dependency chains) and merges them with TwoSum at the end.
+constexpr int kSeedChunk = 256;
+
+__device__ __forceinline__ void dot2_step(double &s, double &c, double a,
+                                          double b) {
+  const double p = a * b;
+  const double e = fma(a, b, -p);
+  const double sn = s + p;
+  const double z = sn - s;
+  c += (s - (sn - z)) + (p - z) + e;
+  s = sn;
+}
+
 __global__ __launch_bounds__(256) void seed_kernel(
     const SeedTask *__restrict__ tasks, const double *__restrict__ X,
     const double *__restrict__ muX, const double *__restrict__ Y,
     const double *__restrict__ muY, int64_t m, double *__restrict__ out) {
+  __shared__ double sxv[256 + kSeedChunk];
+  __shared__ double syv[kSeedChunk];
   const SeedTask task = tasks[blockIdx.x];
   const int j = threadIdx.x;
-  if (j >= task.count) return;
-  const int64_t col = task.row + task.diag0 + j;
-  const double mx = muX[col];
+  const bool live = j < task.count;
+  const int64_t col0 = task.row + task.diag0;
+  const double mx = live ? muX[col0 + j] : 0.0;
   const double my = muY[task.row];
-  const double *x = X + col;
-  const double *y = Y + task.row;
-  double s = 0, c = 0;
-  for (int64_t t = 0; t < m; ++t) {
-    const double a = y[t] - my;
-    const double b = x[t] - mx;
-    const double p = a * b;
-    const double e = fma(a, b, -p);
-    const double sn = s + p;
-    const double z = sn - s;
-    c += (s - (sn - z)) + (p - z) + e;
-    s = sn;
+  double s[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t t0 = 0; t0 < m; t0 += kSeedChunk) {
+    const int cl = (int)(m - t0 < kSeedChunk ? m - t0 : kSeedChunk);
+    __syncthreads();
+    // exactly the samples the task's diagonals read
+    for (int q = j; q < task.count + cl - 1; q += 256)
+      sxv[q] = X[col0 + t0 + q];
+    for (int q = j; q < cl; q += 256) syv[q] = Y[task.row + t0 + q] - my;
+    __syncthreads();
+    if (live) {
+      int t = 0;
+      for (; t + 4 <= cl; t += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          dot2_step(s[u], c[u], syv[t + u], sxv[j + t + u] - mx);
+      }
+      for (; t < cl; ++t) dot2_step(s[0], c[0], syv[t], sxv[j + t] - mx);
+    }
   }
-  out[task.out_offset + j] = s + c;
+  if (!live) return;
+  double sum = s[0], err = c[0] + c[1] + c[2] + c[3];
+#pragma unroll
+  for (int u = 1; u < 4; ++u) {
+    const double sn = sum + s[u];
+    const double z = sn - sum;
+    err += (sum - (sn - z)) + (s[u] - z);
+    sum = sn;
+  }
+  out[task.out_offset + j] = sum + err;
 }
 
 __global__ void mp_to_keys_kernel(uint64_t *p, int64_t n) {
@@ -333,49 +364,54 @@ inline unsigned blocks_for(int64_t n, int bs) {
 //   sum_t x_t (T[j + t] - mu_i) - (mu_j - mu_i) sum_t x_t,  x_t = T[i + t] - mu_i
 // (both factors centred near the windows' level, so a large offset of the
 // series costs no digits); the correlation is formed in the sweep's order,
-// (cov * inorm_col) * inorm_row.  One series per blockIdx.y (stride entries
-// apart).
+// (cov * inorm_col) * inorm_row.  A block's windows, forward partners and
+// backward partners are staged through LDS, kWarmChunk samples of t at a
+// time; each thread slides register windows over the partners so that every
+// staged sample feeds kWarm FMAs.  One series per blockIdx.y (stride entries
+// apart); samples at or past n read as 0 and meet x = 0 or an invalid partner.
 constexpr int kWarm = 8;
+constexpr int kWarmBlock = 256;
+constexpr int kWarmChunk = 256;
+constexpr int kWarmStage = kWarmBlock + kWarmChunk + 2 * kWarm;
 
-// sum_t (T[i + t] - mi) (T[j + t] - mi)
-__device__ inline double warm_dot(const double *T, int64_t i, int64_t j,
-                                  int64_t m, double mi) {
-  double acc = 0.0;
-  for (int64_t t = 0; t < m; ++t) acc = fma(T[i + t] - mi, T[j + t] - mi, acc);
-  return acc;
-}
-
-__global__ __launch_bounds__(256) void warm_kernel(
+__global__ __launch_bounds__(kWarmBlock) void warm_kernel(
     const double *__restrict__ T, const double *__restrict__ mu,
-    const double *__restrict__ inorm, int64_t ns, int64_t m, int64_t e,
-    int64_t stride, uint64_t *keyCol, uint64_t *keyRow) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= ns) return;
+    const double *__restrict__ inorm, int64_t n, int64_t ns, int64_t m,
+    int64_t e, int64_t stride, uint64_t *keyCol, uint64_t *keyRow) {
+  __shared__ double sx[kWarmStage];  // T[i0 + t0 + q]
+  __shared__ double sf[kWarmStage];  // T[i0 + e + t0 + q]
+  __shared__ double sb[kWarmStage];  // T[i0 - e - (kWarm - 1) + t0 + q]
+  const int j = threadIdx.x;
+  const int64_t i0 = blockIdx.x * (int64_t)kWarmBlock;
+  const int64_t i = i0 + j;
   const int64_t o = (int64_t)blockIdx.y * stride;
   T += o;
   mu += o;
   inorm += o;
-  const double ini = inorm[i];
-  if (!(ini == ini)) return;  // a NaN window has no matches
-  const double mi = mu[i];
-  // f[w]: partner i + e + w, b[w]: partner i - e - w
-  double f[kWarm], b[kWarm];
+  const double ini = i < ns ? inorm[i] : nan("");
+  const double mi = i < ns ? mu[i] : 0.0;
+  double f[kWarm], b[kWarm];  // partners i + e + w and i - e - w
 #pragma unroll
   for (int w = 0; w < kWarm; ++w) f[w] = b[w] = 0.0;
-  double sx = 0.0;
-  const bool fwd_all = keyRow && i + e + kWarm - 1 < ns;
-  const bool bwd_all = keyCol && i - e - (kWarm - 1) >= 0;
-  if (fwd_all && bwd_all) {
-    // every partner valid: register windows over the partners' samples,
-    // kWarm steps of t per stage (loads past the window end read the
-    // padded series and meet x = 0)
+  double sxs = 0.0;
+  for (int64_t t0 = 0; t0 < m; t0 += kWarmChunk) {
+    const int cl = (int)(m - t0 < kWarmChunk ? m - t0 : kWarmChunk);
+    __syncthreads();
+    for (int q = j; q < kWarmStage; q += kWarmBlock) {
+      const int64_t ix = i0 + t0 + q, jf = ix + e, jb = ix - e - (kWarm - 1);
+      sx[q] = ix < n ? T[ix] : 0.0;
+      sf[q] = jf < n ? T[jf] : 0.0;
+      sb[q] = jb >= 0 && jb < n ? T[jb] : 0.0;
+    }
+    __syncthreads();
+    // fw[q] = T[i + e + tt + q] - mi, bw[q] = T[i - e - (kWarm-1) + tt + q] - mi
     double fw[2 * kWarm - 1], bw[2 * kWarm - 1];
 #pragma unroll
     for (int q = 0; q < kWarm - 1; ++q) {
-      fw[kWarm + q] = T[i + e + q] - mi;
-      bw[kWarm + q] = T[i - e - (kWarm - 1) + q] - mi;
+      fw[kWarm + q] = sf[j + q] - mi;
+      bw[kWarm + q] = sb[j + q] - mi;
     }
-    for (int64_t t0 = 0; t0 < m; t0 += kWarm) {
+    for (int tt = 0; tt < cl; tt += kWarm) {
 #pragma unroll
       for (int q = 0; q < kWarm - 1; ++q) {
         fw[q] = fw[kWarm + q];
@@ -383,13 +419,13 @@ __global__ __launch_bounds__(256) void warm_kernel(
       }
 #pragma unroll
       for (int q = 0; q < kWarm; ++q) {
-        fw[kWarm - 1 + q] = T[i + e + t0 + kWarm - 1 + q] - mi;
-        bw[kWarm - 1 + q] = T[i - e + t0 + q] - mi;
+        fw[kWarm - 1 + q] = sf[j + tt + kWarm - 1 + q] - mi;
+        bw[kWarm - 1 + q] = sb[j + tt + kWarm - 1 + q] - mi;
       }
 #pragma unroll
       for (int u = 0; u < kWarm; ++u) {
-        const double x = t0 + u < m ? T[i + t0 + u] - mi : 0.0;
-        sx += x;
+        const double x = tt + u < cl ? sx[j + tt + u] - mi : 0.0;
+        sxs += x;
 #pragma unroll
         for (int w = 0; w < kWarm; ++w) {
           f[w] = fma(x, fw[u + w], f[w]);              // T[i+e+w+t] - mi
@@ -397,26 +433,21 @@ __global__ __launch_bounds__(256) void warm_kernel(
         }
       }
     }
-  } else {
-    for (int w = 0; w < kWarm; ++w) {
-      if (keyRow && i + e + w < ns) f[w] = warm_dot(T, i, i + e + w, m, mi);
-      if (keyCol && i - e - w >= 0) b[w] = warm_dot(T, i, i - e - w, m, mi);
-    }
-    for (int64_t t = 0; t < m; ++t) sx += T[i + t] - mi;
   }
+  if (i >= ns || !(ini == ini)) return;  // no window, or one without matches
   uint64_t best_row = 0, best_col = 0;
 #pragma unroll
   for (int w = 0; w < kWarm; ++w) {
     const int64_t jf = i + e + w, jb = i - e - w;
     if (keyRow && jf < ns) {
-      const double c = ((f[w] - (mu[jf] - mi) * sx) * inorm[jf]) * ini;
+      const double c = ((f[w] - (mu[jf] - mi) * sxs) * inorm[jf]) * ini;
       if (c == c) {
         const uint64_t k = make_key((float)c, (uint32_t)jf);
         best_row = k > best_row ? k : best_row;
       }
     }
     if (keyCol && jb >= 0) {
-      const double c = ((b[w] - (mu[jb] - mi) * sx) * ini) * inorm[jb];
+      const double c = ((b[w] - (mu[jb] - mi) * sxs) * ini) * inorm[jb];
       if (c == c) {
         const uint64_t k = make_key((float)c, (uint32_t)jb);
         best_col = k > best_col ? k : best_col;
@@ -538,9 +569,9 @@ hipError_t launch_warm_profile(const double *clean, const double *mean,
                                hipStream_t s) {
   if (ns <= 0 || batch <= 0 || (!keyCol && !keyRow)) return hipSuccess;
   hipLaunchKernelGGL(warm_kernel,
-                     dim3(blocks_for(ns, 256), static_cast<unsigned>(batch)),
-                     dim3(256), 0, s, clean, mean, inorm, ns, m, e, stride,
-                     keyCol, keyRow);
+                     dim3(blocks_for(ns, kWarmBlock), static_cast<unsigned>(batch)),
+                     dim3(kWarmBlock), 0, s, clean, mean, inorm, ns + m - 1, ns,
+                     m, e, stride, keyCol, keyRow);
   return hipGetLastError();
 }
 
