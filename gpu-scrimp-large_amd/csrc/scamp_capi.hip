@@ -905,10 +905,11 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
     double *dg = ar.alloc<double>(batch * stat_len);
     HIP_CHECK(launch_stats(d_T, (int64_t)n, m, clean, clean_len, mean, inorm,
                            df, dg, stat_len, s, batch, (int64_t)n));
+    double *work = ar.alloc<double>(2 * batch * ns);
     HIP_CHECK(hipMemsetAsync(d_grad_T, 0, sizeof(double) * n * batch, s));
     HIP_CHECK(launch_pearson_backward(d_T, (int64_t)n, m, batch, mean, inorm,
                                       stat_len, d_index, d_grad_corr, d_grad_T,
-                                      s));
+                                      work, s));
     HIP_CHECK(hipStreamSynchronize(s));
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
