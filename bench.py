@@ -7,6 +7,7 @@ form's error against FP64.
     python bench.py --calibrate     # copy kernel of known size, for --pmc scaling
     python bench.py --workload selfjoin-n33554432-m4096   # the large self-join
     python bench.py --workload batch-64x8192-m128         # differentiable batched join
+    python bench.py --workload abjoin-2097152x8388608-m256  # AB-join with a planted motif
 
 One step = one complete self-join (profile reset, statistics, exact seeds,
 the diagonal sweep, profile conversion) of the workload below, with the input
@@ -204,6 +205,67 @@ def large_selfjoin(args):
     print(json.dumps(line))
 
 
+def abjoin_large(args):
+    """AB-join of A = 2^21-sample random walk with B = 2^23 samples of three
+    sinusoids plus noise and 200 planted copies of a 256-sample motif
+    (make_abjoin_series), m = 256: 1NN_INDEX with both profiles (A's
+    columns and B's rows kept separate) and SUM_THRESH at 0.9, each timed once
+    in the mixed form and in the plain FP64 form (the error reference)."""
+    import torch
+    from scamp_amd import _capi as C
+    m, thr = 256, 0.9
+    a, b, _ = make_abjoin_series()
+    na, nb = a.size, b.size
+    nsa, nsb = na - m + 1, nb - m + 1
+    pairs = nsa * nsb
+    dev = torch.device("cuda:0")
+    a_d, b_d = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+    init = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
+    out, prof = {}, {}
+    for ptype in ("1NN_INDEX", "SUM_THRESH"):
+        for prec in ("mixed", "double"):
+            if ptype == "1NN_INDEX":
+                pa = torch.full((nsa,), init, dtype=torch.int64, device=dev)
+                pb = torch.full((nsb,), init, dtype=torch.int64, device=dev)
+            else:
+                pa = torch.zeros(nsa, dtype=torch.float64, device=dev)
+                pb = torch.zeros(nsb, dtype=torch.float64, device=dev)
+            params = C.make_params(m, ptype, precision=prec, has_b=True,
+                                   keep_rows_separate=True, threshold=thr)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            C.join_device(a_d.data_ptr(), na, b_d.data_ptr(), nb, params, pa.data_ptr(),
+                          pb.data_ptr())
+            dt = time.perf_counter() - t0
+            key = f"{prec}_{ptype.lower()}"
+            out[key] = {"seconds": round(dt, 4), "pairs_per_s": pairs / dt}
+            prof[key] = (pa.cpu().numpy(), pb.cpu().numpy())
+            print(f"{key}: {dt:.3f} s, {pairs / dt:.4g} pairs/s", file=sys.stderr, flush=True)
+    err = {}
+    for side, k in (("columns_a", 0), ("rows_b", 1)):
+        cm, im = C.split_mp_entry(prof["mixed_1nn_index"][k].view(np.uint64))
+        cd, idd = C.split_mp_entry(prof["double_1nn_index"][k].view(np.uint64))
+        ok = cd > -1
+        err[side] = {"max_abs_dcorr": float(np.max(np.abs(cm[ok].astype(np.float64) - cd[ok]))),
+                     "index_agreement": float(np.mean(im[ok] == idd[ok]))}
+        sm, sd = prof["mixed_sum_thresh"][k], prof["double_sum_thresh"][k]
+        err[side]["sum_thresh_max_rel_err"] = float(
+            np.max(np.abs(sm - sd) / np.maximum(np.abs(sd), 1.0)))
+    line = {
+        "metric": "AB-join pairs/sec (2^21 x 2^23, m=256): mixed form, 1NN_INDEX + SUM_THRESH",
+        "value": out["mixed_1nn_index"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": out["mixed_1nn_index"]["seconds"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "AB-join: A random walk n=2097152 (PCG64 seed 3), B three "
+                               "sinusoids + N(0,0.3) noise n=8388608 (seed 4) with 200 planted "
+                               "256-sample motifs; m=256, both profiles, SUM_THRESH at 0.9",
+                   "na": na, "nb": nb, "window": m, "pairs": pairs},
+        "joins": out, "mixed_vs_fp64": err,
+    }
+    print(json.dumps(line))
+
+
 def batch_selfjoin(args):
     """The differentiable batched self-join: 64 float32 random walks of 8192
     samples (PCG64 seeds 100..163), m = 128.  One step = forward (one batched
@@ -291,7 +353,7 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the copy kernel used to scale HBM counters")
     ap.add_argument("--workload", choices=("selfjoin-n4194304-m1024", "selfjoin-n33554432-m4096",
-                                           "batch-64x8192-m128"),
+                                           "batch-64x8192-m128", "abjoin-2097152x8388608-m256"),
                     default="selfjoin-n4194304-m1024")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -306,6 +368,9 @@ def main():
         return
     if args.workload == "batch-64x8192-m128":
         batch_selfjoin(args)
+        return
+    if args.workload == "abjoin-2097152x8388608-m256":
+        abjoin_large(args)
         return
 
     import torch
