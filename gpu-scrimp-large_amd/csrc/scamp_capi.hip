@@ -232,9 +232,11 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
                       (uint64_t)slots) * kDiagsPerLane;
     total_rows += rows0[b];
   }
+  // (a batch fills the GPU with its series' bands alone once there are
+  // enough of them; the chunk length itself depends on one series only)
   const int64_t wave_slots = (int64_t)ctx->num_cus * 12;
   int64_t R = std::numeric_limits<int64_t>::max();
-  if (nbands < 2 * wave_slots) {
+  if (nbands * nbatch < 2 * wave_slots) {
     const int64_t floor_rows = std::max<int64_t>(512, 4 * m);
     R = std::max(floor_rows, (total_rows + 4 * wave_slots - 1) / (4 * wave_slots));
     R = (R + 15) / 16 * 16;

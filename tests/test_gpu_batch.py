@@ -1,10 +1,13 @@
 """The batched self-join (scamp_hip_selfjoin_batch): `batch` series in one
 launch must give exactly what one scamp_hip_join_device self-join per series
-gives (same statistics, seeds, band rows and merges, so bit-identical
-profiles), for FP64 and the mixed form, 1NN_INDEX and 1NN, with NaN runs,
-with prior profiles that must be kept where they are better, and for series
-shorter than one 512-diagonal band as well as several bands long.  The
-per-series join is itself checked against the oracle by the parity tests."""
+gives (batches this small cut their bands into the same row chunks as a single
+series, so the statistics, seeds, rows and merges are the same and the
+profiles bit-identical), for FP64 and the mixed form, 1NN_INDEX and 1NN, with
+NaN runs, with prior profiles that must be kept where they are better, and for
+series shorter than one 512-diagonal band as well as several bands long.  A
+batch with enough bands to fill the GPU is not cut into chunks; it is checked
+against the per-series joins to FP64 rounding.  The per-series join is itself
+checked against the oracle by the parity tests."""
 import ctypes
 
 import numpy as np
@@ -103,3 +106,28 @@ def test_batch_argument_errors():
     rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 2,
                                       ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
     assert rc == 16
+
+
+def test_large_batch_without_chunks_matches_per_series():
+    """1600 series of 2500 samples (m = 64): 5 bands each, 8000 bands in all,
+    enough to fill the GPU without cutting bands into row chunks, while each
+    series alone is cut.  Correlations agree to FP32 rounding of values that
+    differ by FP64 rounding; an index may differ only at a tie."""
+    B, n, m = 1600, 2500, 64
+    rng = np.random.default_rng(5)
+    X = np.cumsum(rng.standard_normal((B, n)), axis=1)
+    ns = n - m + 1
+    prior = torch.tensor(np.tile(C.init_profile("1NN_INDEX", ns), (B, 1)).view(np.int64),
+                         device="cuda")
+    got = _batched(X, m, "1NN_INDEX", "double", prior).cpu().numpy().view(np.uint64)
+    pick = rng.choice(B, 12, replace=False)
+    want = _per_series(X[pick], m, "1NN_INDEX", "double",
+                       prior[:12].clone()).cpu().numpy().view(np.uint64)
+    for q, b in enumerate(pick):
+        cg, ig = C.split_mp_entry(got[b])
+        cw, iw = C.split_mp_entry(want[q])
+        assert np.max(np.abs(cg.astype(np.float64) - cw)) <= 1.2e-7
+        diff = ig != iw
+        assert np.mean(diff) < 1e-3
+        if diff.any():
+            assert np.max(np.abs(cg[diff].astype(np.float64) - cw[diff])) <= 1.2e-7
