@@ -357,8 +357,8 @@ def main():
                     default="selfjoin-n4194304-m1024")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the join on a single "
-              "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs each join on a single "
+              "MI355X (sharding a join across GPUs is not implemented).", file=sys.stderr)
         sys.exit(2)
     if args.calibrate:
         calibrate()

@@ -183,7 +183,7 @@ int main(int argc, char **argv) {
     return 1;
   }
   if (B("no_gpu")) {
-    printf("Error: this build computes on the GPU only (--no_gpu is not supported)\n");
+    printf("Error: this engine computes on the GPU only (--no_gpu is not supported)\n");
     return 1;
   }
   if (S("input_a_file_name").empty()) {

@@ -1,7 +1,7 @@
 """ctypes access to the oracle libraries (test infrastructure only).
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
-import this module, and only as the checker.  It wraps
+import this module, and only to check results against.  It wraps
 
 * ``liboracle.so``         -- the C restatement of SCAMP's CPU double path
   (oracle/scamp_oracle.c, which cites the reference's files line by line);

@@ -8,7 +8,7 @@
 // calls SCAMP::do_SCAMP(args, devices={}, num_threads)
 // (src/SCAMP.cpp:279-334).
 //
-// TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline).
+// TEST INFRASTRUCTURE ONLY (the tests' reference results and the CPU baseline).
 #include <cstdint>
 #include <cstring>
 #include <exception>

@@ -1,7 +1,7 @@
 /*
  * Oracle: CPU restatement of SCAMP's CPU double path.
  *
- * TEST INFRASTRUCTURE ONLY -- the checker, never the thing measured or
+ * TEST INFRASTRUCTURE ONLY -- results are checked against it; it is never the thing measured or
  * shipped.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg load it.
  *

@@ -4,7 +4,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product links or calls this code;
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
- * and only as the checker.
+ * and only to check results against.
  *
  * Every function cites the reference file:line it follows (paths relative to
  * the GPU-SCRIMP-LARGE source tree).
