@@ -126,6 +126,7 @@ struct Sweep {
   static constexpr int kRows = KIND == kNN ? 2 * K : K;
   DT cov[K];
   DT hi[K];
+  uint64_t *s_kq = nullptr;  // ALL_NEIGHBORS: per-lane candidate slots (LDS)
   // SUM_THRESH rows: this lane's partial sum of each row of the current block
   double rsum[K];
   Slot<DT> S[K];         // column slots, indexed by column mod K
@@ -518,16 +519,32 @@ struct Sweep {
       }
     }
     if (COLS) {
+      // The lane's candidates go to its own LDS slots first, so that one
+      // insert loop (over the set bits) serves all K diagonals: the insert
+      // code then exists once per row rather than K times, which keeps the
+      // kernel's register count down.  The slots then carry each list's new
+      // minimum back to the columns' thresholds.
+      uint64_t *q = s_kq + lane * K;
+      uint32_t mask = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        Slot<DT> &s = S[(r + k) % K];
-        if (c[k] >= s.t) {
-          const int64_t j = i + d0 + k;
-          const uint64_t key = make_key((float)c[k], static_cast<uint32_t>(i));
-          const uint64_t nm = knn_insert(a.knnX + j * kk, kk, key);
-          atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + j),
-                    static_cast<unsigned long long>(nm));
-          const DT tn = Thresh<DT>::from_key_hi(static_cast<uint32_t>(nm >> 32));
+        const bool h = c[k] >= S[(r + k) % K].t;
+        q[k] = h ? make_key((float)c[k], static_cast<uint32_t>(i)) : 0;
+        mask |= (h ? 1u : 0u) << k;
+      }
+      for (uint32_t mk = mask; mk != 0; mk &= mk - 1) {
+        const int k = __builtin_ctz(mk);
+        const int64_t j = i + d0 + k;
+        const uint64_t nm = knn_insert(a.knnX + j * kk, kk, q[k]);
+        atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + j),
+                  static_cast<unsigned long long>(nm));
+        q[k] = nm;
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if ((mask >> k) & 1u) {
+          Slot<DT> &s = S[(r + k) % K];
+          const DT tn = Thresh<DT>::from_key_hi(static_cast<uint32_t>(q[k] >> 32));
           s.t = s.t > tn ? s.t : tn;
         }
       }
@@ -781,9 +798,13 @@ struct Sweep {
 // VGPRs) and 4 for FP32 (<= 128).  With 16-row blocks the compiler would
 // otherwise use more registers for look-ahead and drop a wave; the few
 // spills the targets cost are cheaper than the lost wave (measured).
+// ALL_NEIGHBORS in FP32: 3 waves (12 % faster than the 2 the compiler
+// picks); in FP64 the same target spills heavily (4x slower), so none.
 template <typename DT, int KIND>
 constexpr int sweep_waves_per_eu() {
-  return KIND == kNN ? (sizeof(DT) == 4 ? 4 : 3) : 1;
+  if (KIND == kNN) return sizeof(DT) == 4 ? 4 : 3;
+  if (KIND == kKnn) return sizeof(DT) == 4 ? 3 : 1;
+  return 1;
 }
 
 template <typename DT, int KIND, bool COLS, bool ROWS>
@@ -795,6 +816,7 @@ void sweep_kernel(SweepArgs a) {
   // FP32 form only: anchor staging (unreferenced, hence unallocated, in FP64)
   __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
   __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
+  __shared__ uint64_t s_kq[KIND == kKnn ? kBand : 1];
   const int lane = threadIdx.x;
   const unsigned int nitems = (unsigned int)a.nitems;
   for (;;) {
@@ -825,6 +847,7 @@ void sweep_kernel(SweepArgs a) {
     }
     const int64_t D0 = la.d_base + (int64_t)band * kBand;
     Sweep<DT, KIND, COLS, ROWS> sw(la, lane, D0);
+    sw.s_kq = s_kq;
     for (int seg = 0; seg < 2; ++seg) {
       const int64_t rs = seg == 0 ? it->rs : it->rs2;
       const int64_t re = seg == 0 ? it->re : it->re2;
