@@ -824,14 +824,18 @@ void sweep_kernel(SweepArgs a) {
     if (lane == 0) b = atomicAdd(a.counter, 1u);
     b = (unsigned int)__builtin_amdgcn_readfirstlane((int)b);
     if (b >= nitems) break;
-    // the item is read where each field is needed (not held in registers
-    // across the first segment)
+    // FP64: the item is read where each field is needed (holding it in
+    // registers across the first segment costs the FP64 kernel spills);
+    // FP32: read whole up front (measured faster there)
     const SweepItem *it = a.items + b;
-    const int32_t band = it->band;
+    SweepItem iv{};
+    if (sizeof(DT) == 4) iv = *it;
+    const int32_t band = sizeof(DT) == 4 ? iv.band : it->band;
     // batched joins: the item's series, seen through offset pointers
     SweepArgs la = a;
     if (a.nbatch > 1) {
-      const int64_t o = (int64_t)it->series * a.batch_stride;
+      const int64_t o =
+          (int64_t)(sizeof(DT) == 4 ? iv.series : it->series) * a.batch_stride;
       la.dfX = static_cast<const DT *>(a.dfX) + o;
       la.dgX = static_cast<const DT *>(a.dgX) + o;
       la.inX = static_cast<const DT *>(a.inX) + o;
@@ -849,8 +853,10 @@ void sweep_kernel(SweepArgs a) {
     Sweep<DT, KIND, COLS, ROWS> sw(la, lane, D0);
     sw.s_kq = s_kq;
     for (int seg = 0; seg < 2; ++seg) {
-      const int64_t rs = seg == 0 ? it->rs : it->rs2;
-      const int64_t re = seg == 0 ? it->re : it->re2;
+      const int64_t rs = sizeof(DT) == 4 ? (seg == 0 ? iv.rs : iv.rs2)
+                                         : (seg == 0 ? it->rs : it->rs2);
+      const int64_t re = sizeof(DT) == 4 ? (seg == 0 ? iv.re : iv.re2)
+                                         : (seg == 0 ? it->re : it->re2);
       if (rs >= re) continue;
       const int64_t dlast = D0 + kBand < la.d_hi ? D0 + kBand - 1 : la.d_hi - 1;
       // all of the band's diagonals are valid on rows below rfull
