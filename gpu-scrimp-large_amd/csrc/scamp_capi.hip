@@ -613,14 +613,16 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   const bool mp = pt == SCAMP_HIP_1NN_INDEX;
   HIP_CHECK(launch_profile_in(d_prof, mp, key, ns, batch, seg, s));
 
-  // diagonals [m/4, ns) of each series, rows and columns into one profile
-  // (src/tile.cpp:12-38 keeps |i - j| < m/4 out), after the same warm start
-  // as a single self-join
+  // diagonals [m/4, ns) of each series (src/tile.cpp:12-38 keeps |i - j| <
+  // m/4 out), rows and columns into one profile as the flags ask, after the
+  // same warm start as a single self-join
+  const bool cols = p->computing_columns != 0, rows = p->computing_rows != 0;
   HIP_CHECK(launch_warm_profile(A.clean, A.mean, A.inorm, ns, m, m / 4, batch,
-                                seg, key, key, s));
+                                seg, cols ? key : nullptr, rows ? key : nullptr,
+                                s));
   float sweep_ms = 0;
   for (auto &r : cut(m / 4, ns, 0, 0)) {
-    Pass ps{&A, &A, r.first, r.second, true, true, key, key,
+    Pass ps{&A, &A, r.first, r.second, cols, rows, key, key,
             nullptr, nullptr, nullptr, nullptr};
     run_pass(ctx, ar, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor, 0,
              &sweep_ms, batch, seg, ns);
@@ -736,6 +738,9 @@ int scamp_hip_selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T,
   if (batch < 1) return fail(SCAMP_HIP_INVALID_ARGS, "batch must be >= 1");
   if (p->has_b)
     return fail(SCAMP_HIP_INVALID_ARGS, "a batched join is a self-join");
+  if (p->keep_rows_separate)
+    return fail(SCAMP_HIP_INVALID_ARGS,
+                "a batched self-join keeps rows and columns in one profile");
   if (p->profile_type != SCAMP_HIP_1NN_INDEX &&
       p->profile_type != SCAMP_HIP_1NN)
     return fail(SCAMP_HIP_UNIMPLEMENTED,

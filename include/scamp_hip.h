@@ -133,8 +133,10 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
  * python/SCAMP_python.cpp:67-110).  d_T holds `batch` float64 series of n
  * samples back to back (DEVICE); d_profiles holds batch * (n - m + 1)
  * entries of a 1NN_INDEX (uint64 mp_entry) or 1NN (float) profile, read as
- * the prior best-so-far and updated in place.  Equals `batch` calls of
- * scamp_hip_join_device with has_b = 0 (bit for bit while the batch is small
+ * the prior best-so-far and updated in place; computing_rows/columns apply
+ * as in a self-join, keep_rows_separate is refused (one profile per series).
+ * Equals `batch` calls of scamp_hip_join_device with has_b = 0 (bit for bit
+ * while the batch is small
  * enough that its bands are cut into row chunks like a single series'; up to
  * FP64 rounding of the recurrence beyond); fills the GPU when each series
  * alone has too few diagonals to.  Blocks until done. */

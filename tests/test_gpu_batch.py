@@ -106,6 +106,32 @@ def test_batch_argument_errors():
     rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 2,
                                       ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
     assert rc == 16
+    p = C.make_params(50, keep_rows_separate=True)
+    rc = lib.scamp_hip_selfjoin_batch(ctx.handle, ctypes.c_void_p(T.data_ptr()), 1000, 2,
+                                      ctypes.byref(p), ctypes.c_void_p(out.data_ptr()))
+    assert rc == 16 and "one profile" in C.last_error()
+
+
+@pytest.mark.parametrize("rows,cols", [(False, True), (True, False)])
+def test_batch_honours_row_and_column_flags(rows, cols):
+    """A self-join computing only columns (or only rows) keeps, for each
+    subsequence, its best match on one side: the batch must agree with the
+    single joins under the same flags."""
+    B, n, m = 4, 1500, 40
+    X = np.stack([random_walk(n, 900 + b, nan_runs=b % 2) for b in range(B)])
+    ns = n - m + 1
+    prior = torch.tensor(np.tile(C.init_profile("1NN_INDEX", ns), (B, 1)).view(np.int64),
+                         device="cuda")
+    T = torch.tensor(X, dtype=torch.float64, device="cuda")
+    params = C.make_params(m, computing_rows=rows, computing_columns=cols)
+    want = prior.clone()
+    torch.cuda.synchronize()
+    for b in range(B):
+        C.join_device(T[b].data_ptr(), n, 0, 0, params, want[b].data_ptr())
+    got = prior.clone()
+    C.selfjoin_batch_device(T.data_ptr(), n, B, params, got.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
 
 
 def test_large_batch_without_chunks_matches_per_series():
