@@ -9,6 +9,15 @@
 // (the centring drops out: both vectors sum to zero).  Each profile entry
 // adds g_i * dr/dx into T[i .. i+m) and g_i * dr/dy into T[I_i .. I_i+m).
 // This path has no counterpart in the reference (SCAMP has no gradients).
+//
+// Every sample's gradient is gathered by one thread in a fixed order, with no
+// atomics, so the result is the same bit for bit from run to run:
+//   1. one wavefront per profile entry: r and three coefficients;
+//   2. the entries sorted by partner window (radix sort of (partner, entry));
+//   3. per sample t, the self half over the windows i that contain t, then
+//      the partner half over the entries whose partner window contains t.
+#include <hipcub/hipcub.hpp>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -19,29 +28,41 @@ constexpr int kBwdBlock = 256;
 constexpr int kBwdWaves = kBwdBlock / kWave;
 constexpr int kBwdChunk = 512;  // entries staged through LDS at a time
 
-// Pass 1, one wavefront per (series b, entry i): r = <x^, y^> from the
-// centred windows (means and inverse norms from the statistics kernels, one
-// row of `stat_stride` entries per series); the partner half of the gradient,
-// g (x^ - r y^) / |y~|, into T[j .. j+m) with FP64 atomics; and the self
-// half's coefficients for pass 2, P_i = g/(|x~||y~|) and Q_i = g r/|x~|^2
-// (both 0 for an entry without a gradient).
-__global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
+// Per entry e = b * ns + i (partner j = I_i, g = dL/dP_i):
+//   P = g / (|x~| |y~|), Q = g r / |x~|^2, R = g r / |y~|^2
+// The self half at position k of window i is P (T[j+k] - mu_j) - Q (T[i+k] -
+// mu_i); the partner half at position k of window j is P (T[i+k] - mu_i) -
+// R (T[j+k] - mu_j).  Entries without a gradient (no match, g = 0, NaN or
+// flat window) get P = Q = R = 0 and sort after all others.
+struct Coef {
+  double P, Q, R;
+};
+
+// Pass 1, one wavefront per (series b, entry i).  Means and inverse norms
+// come from the statistics kernels (one row of `stat_stride` entries per
+// series).
+__global__ __launch_bounds__(kBwdBlock) void bwd_coef_kernel(
     const double *__restrict__ T, int64_t n, int64_t m, int64_t ns,
-    const double *__restrict__ mean, const double *__restrict__ inorm,
-    int64_t stat_stride, const int32_t *__restrict__ index,
-    const double *__restrict__ grad_p, double *__restrict__ grad_t,
-    double *__restrict__ coef) {
+    int64_t batch, const double *__restrict__ mean,
+    const double *__restrict__ inorm, int64_t stat_stride,
+    const int32_t *__restrict__ index, const double *__restrict__ grad_p,
+    Coef *__restrict__ coef, uint64_t *__restrict__ key,
+    uint64_t *__restrict__ val) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t b = blockIdx.y;
   const int64_t i = (int64_t)blockIdx.x * kBwdWaves + threadIdx.x / kWave;
   if (i >= ns) return;
-  double *P = coef + 2 * (b * ns + i);
-  const int64_t j = index[b * ns + i];
-  const double g = grad_p[b * ns + i];
+  const int64_t e = b * ns + i;
+  const int64_t j = index[e];
+  const double g = grad_p[e];
   const double ni = inorm[b * stat_stride + i];
   const double nj = j >= 0 && j < ns ? inorm[b * stat_stride + j] : nan("");
   if (j < 0 || j >= ns || g == 0.0 || !(ni == ni) || !(nj == nj)) {
-    if (lane == 0) P[0] = P[1] = 0.0;  // NaN / flat window: no gradient
+    if (lane == 0) {
+      coef[e] = Coef{0.0, 0.0, 0.0};
+      key[e] = (uint64_t)(batch * ns);  // after every real partner
+      val[e] = (uint64_t)e;
+    }
     return;
   }
   const double *Tb = T + b * n;
@@ -50,28 +71,20 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
   for (int64_t k = lane; k < m; k += kWave)
     part = fma((Tb[i + k] - mi) * ni, (Tb[j + k] - mj) * nj, part);
   const double r = wave_sum(part);
-  double *Gb = grad_t + b * n;
-  for (int64_t k = lane; k < m; k += kWave) {
-    const double xh = (Tb[i + k] - mi) * ni;
-    const double yh = (Tb[j + k] - mj) * nj;
-    atomicAdd(Gb + j + k, g * (xh - r * yh) * nj);
-  }
   if (lane == 0) {
-    P[0] = g * ni * nj;
-    P[1] = g * r * ni * ni;
+    coef[e] = Coef{g * ni * nj, g * r * ni * ni, g * r * nj * nj};
+    key[e] = (uint64_t)(b * ns + j);
+    val[e] = (uint64_t)e;
   }
 }
 
-// Pass 2, one thread per sample t: the self half, a gather over the windows
-// that contain t (entries i in [t-m+1, t]):
-//   grad[t] += sum_i P_i (T[j_i + t - i] - mu_j) - Q_i (T[t] - mu_i)
-// which is g (y^ - r x^) / |x~| at position t - i of window i.  Each sample is
-// written by one thread, after pass 1 on the same stream: no atomics.  The
-// block's entries (coefficients, partners, both means) are staged in LDS.
+// Pass 3a, one thread per sample t: the self half, a gather over the windows
+// that contain t (entries i in [t-m+1, t]); writes grad_t.  The block's
+// entries (coefficients, partners, both means) are staged in LDS.
 __global__ __launch_bounds__(kBwdBlock) void bwd_self_kernel(
     const double *__restrict__ T, int64_t n, int64_t m, int64_t ns,
     const double *__restrict__ mean, int64_t stat_stride,
-    const int32_t *__restrict__ index, const double *__restrict__ coef,
+    const int32_t *__restrict__ index, const Coef *__restrict__ coef,
     double *__restrict__ grad_t) {
   __shared__ double sP[kBwdChunk], sQ[kBwdChunk], sMi[kBwdChunk], sMj[kBwdChunk];
   __shared__ int32_t sJ[kBwdChunk];
@@ -89,11 +102,11 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_self_kernel(
     __syncthreads();
     for (int q = threadIdx.x; q < cn; q += kBwdBlock) {
       const int64_t e = c0 + q;
-      const double p = coef[2 * (b * ns + e)], qq = coef[2 * (b * ns + e) + 1];
+      const Coef cf = coef[b * ns + e];
       const int32_t j = index[b * ns + e];
-      const bool live = p != 0.0 || qq != 0.0;
-      sP[q] = p;
-      sQ[q] = qq;
+      const bool live = cf.P != 0.0 || cf.Q != 0.0;
+      sP[q] = cf.P;
+      sQ[q] = cf.Q;
       sJ[q] = live ? j : -1;
       sMi[q] = mean[b * stat_stride + e];
       sMj[q] = live ? mean[b * stat_stride + j] : 0.0;
@@ -112,27 +125,162 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_self_kernel(
       }
     }
   }
+  if (t < n) grad_t[b * n + t] = acc;
+}
+
+// First position in keys[0, len) holding a value >= v.
+__device__ int64_t lower_bound_u64(const uint64_t *keys, int64_t len, uint64_t v) {
+  int64_t lo = 0, hi = len;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (keys[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Pass 3b, one thread per sample t: the partner half, a gather over the
+// entries whose partner window contains t (partners j in [t-m+1, t]), found
+// in the entries sorted by (series, partner); adds into grad_t.
+__global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
+    const double *__restrict__ T, int64_t n, int64_t m, int64_t ns,
+    const double *__restrict__ mean, int64_t stat_stride,
+    const Coef *__restrict__ coef, const uint64_t *__restrict__ skey,
+    const uint64_t *__restrict__ sval, int64_t nent,
+    double *__restrict__ grad_t) {
+  __shared__ double sP[kBwdChunk], sR[kBwdChunk], sMi[kBwdChunk], sMj[kBwdChunk];
+  __shared__ int32_t sI[kBwdChunk], sJ[kBwdChunk];
+  __shared__ int64_t range[2];
+  const int64_t b = blockIdx.y;
+  const int64_t t0 = (int64_t)blockIdx.x * kBwdBlock;
+  const int64_t t = t0 + threadIdx.x;
+  const double *Tb = T + b * n;
+  const double tt = t < n ? Tb[t] : 0.0;
+  // partners reaching the block's samples: [t0-m+1, t0+255] within [0, ns)
+  const int64_t jlo = t0 - m + 1 > 0 ? t0 - m + 1 : 0;
+  const int64_t jhi = t0 + kBwdBlock - 1 < ns - 1 ? t0 + kBwdBlock - 1 : ns - 1;
+  if (threadIdx.x == 0) {
+    range[0] = lower_bound_u64(skey, nent, (uint64_t)(b * ns + jlo));
+    range[1] = lower_bound_u64(skey, nent, (uint64_t)(b * ns + jhi + 1));
+  }
+  __syncthreads();
+  const int64_t p0 = range[0], p1 = range[1];
+  double acc = 0.0;
+  for (int64_t c0 = p0; c0 < p1; c0 += kBwdChunk) {
+    const int cn = (int)(p1 - c0 < kBwdChunk ? p1 - c0 : kBwdChunk);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cn; q += kBwdBlock) {
+      const int64_t e = (int64_t)sval[c0 + q];
+      const int64_t i = e - b * ns;
+      const int64_t j = (int64_t)skey[c0 + q] - b * ns;
+      const Coef cf = coef[e];
+      sP[q] = cf.P;
+      sR[q] = cf.R;
+      sI[q] = (int32_t)i;
+      sJ[q] = (int32_t)j;
+      sMi[q] = mean[b * stat_stride + i];
+      sMj[q] = mean[b * stat_stride + j];
+    }
+    __syncthreads();
+    if (t < n) {
+      // the chunk's entries with partner j in [t-m+1, t] (sJ is sorted)
+      int a = 0, z = cn;
+      const int64_t want_lo = t - m + 1;
+      while (a < z) {
+        const int mid = (a + z) / 2;
+        if (sJ[mid] < want_lo) a = mid + 1; else z = mid;
+      }
+      for (int q = a; q < cn && sJ[q] <= t; ++q) {
+        const int64_t i = sI[q], j = sJ[q];
+        acc += sP[q] * (Tb[i + t - j] - sMi[q]) - sR[q] * (tt - sMj[q]);
+      }
+    }
+  }
   if (t < n) grad_t[b * n + t] += acc;
 }
 
+// Workspace: coefficients, sort keys and values in and out, the sort's own
+// temporary storage.
+struct BwdWork {
+  Coef *coef;
+  uint64_t *key_in, *key_out, *val_in, *val_out;
+  void *tmp;
+  size_t tmp_bytes;
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+hipError_t sort_bytes(int64_t nent, int end_bit, size_t *bytes) {
+  return hipcub::DeviceRadixSort::SortPairs(
+      nullptr, *bytes, static_cast<const uint64_t *>(nullptr),
+      static_cast<uint64_t *>(nullptr), static_cast<const uint64_t *>(nullptr),
+      static_cast<uint64_t *>(nullptr), static_cast<int>(nent), 0, end_bit);
+}
+
+int key_bits(int64_t nent) {
+  int bits = 1;
+  while (bits < 63 && (int64_t(1) << bits) <= nent) ++bits;  // keys <= nent
+  return bits;
+}
+
+BwdWork carve(void *base, int64_t nent, size_t tmp_bytes) {
+  char *p = static_cast<char *>(base);
+  BwdWork w;
+  w.coef = reinterpret_cast<Coef *>(p);
+  p += align256(sizeof(Coef) * nent);
+  w.key_in = reinterpret_cast<uint64_t *>(p);
+  p += align256(8 * nent);
+  w.key_out = reinterpret_cast<uint64_t *>(p);
+  p += align256(8 * nent);
+  w.val_in = reinterpret_cast<uint64_t *>(p);
+  p += align256(8 * nent);
+  w.val_out = reinterpret_cast<uint64_t *>(p);
+  p += align256(8 * nent);
+  w.tmp = p;
+  w.tmp_bytes = tmp_bytes;
+  return w;
+}
+
 }  // namespace
+
+size_t pearson_backward_workspace(int64_t n, int64_t m, int64_t batch) {
+  const int64_t nent = batch * (n - m + 1);
+  if (nent <= 0) return 0;
+  size_t tmp = 0;
+  if (sort_bytes(nent, key_bits(nent), &tmp) != hipSuccess) return 0;
+  return align256(sizeof(Coef) * nent) + 4 * align256(8 * nent) + align256(tmp);
+}
 
 hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
                                    int64_t batch, const double *mean,
                                    const double *inorm, int64_t stat_stride,
                                    const int32_t *index, const double *grad_p,
-                                   double *grad_t, double *work, hipStream_t s) {
+                                   double *grad_t, void *work, hipStream_t s) {
   const int64_t ns = n - m + 1;
   if (ns <= 0 || batch <= 0) return hipSuccess;
+  const int64_t nent = batch * ns;
+  const int bits = key_bits(nent);
+  size_t tmp = 0;
+  hipError_t err = sort_bytes(nent, bits, &tmp);
+  if (err != hipSuccess) return err;
+  BwdWork w = carve(work, nent, tmp);
   const unsigned nb = static_cast<unsigned>(batch);
-  hipLaunchKernelGGL(bwd_partner_kernel,
+  hipLaunchKernelGGL(bwd_coef_kernel,
                      dim3(static_cast<unsigned>((ns + kBwdWaves - 1) / kBwdWaves), nb),
-                     dim3(kBwdBlock), 0, s, T, n, m, ns, mean, inorm, stat_stride,
-                     index, grad_p, grad_t, work);
-  hipLaunchKernelGGL(bwd_self_kernel,
-                     dim3(static_cast<unsigned>((n + kBwdBlock - 1) / kBwdBlock), nb),
-                     dim3(kBwdBlock), 0, s, T, n, m, ns, mean, stat_stride, index,
-                     work, grad_t);
+                     dim3(kBwdBlock), 0, s, T, n, m, ns, batch, mean, inorm,
+                     stat_stride, index, grad_p, w.coef, w.key_in, w.val_in);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  err = hipcub::DeviceRadixSort::SortPairs(w.tmp, w.tmp_bytes, w.key_in, w.key_out,
+                                           w.val_in, w.val_out, static_cast<int>(nent),
+                                           0, bits, s);
+  if (err != hipSuccess) return err;
+  const dim3 grid(static_cast<unsigned>((n + kBwdBlock - 1) / kBwdBlock), nb);
+  hipLaunchKernelGGL(bwd_self_kernel, grid, dim3(kBwdBlock), 0, s, T, n, m, ns,
+                     mean, stat_stride, index, w.coef, grad_t);
+  hipLaunchKernelGGL(bwd_partner_kernel, grid, dim3(kBwdBlock), 0, s, T, n, m, ns,
+                     mean, stat_stride, w.coef, w.key_out, w.val_out, nent, grad_t);
   return hipGetLastError();
 }
 

@@ -69,15 +69,17 @@ hipError_t launch_warm_profile(const double *clean, const double *mean,
                                hipStream_t s);
 
 // ---- backward of the differentiable join (backward_kernel.hip) ------------
-// grad_t[b, :] += sum over profile entries i of grad_p[b, i] * d corr(i, I_i)
+// grad_t[b, :] = sum over profile entries i of grad_p[b, i] * d corr(i, I_i)
 // / d T[b, :] for `batch` series of length n stored back to back; mean/inorm
-// rows have `stat_stride` entries per series; `work` holds 2 * batch *
-// (n - m + 1) doubles.
+// rows have `stat_stride` entries per series; `work` holds
+// pearson_backward_workspace(n, m, batch) bytes (batch * (n - m + 1) must be
+// below 2^31).  Deterministic: no atomics.
+size_t pearson_backward_workspace(int64_t n, int64_t m, int64_t batch);
 hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
                                    int64_t batch, const double *mean,
                                    const double *inorm, int64_t stat_stride,
                                    const int32_t *index, const double *grad_p,
-                                   double *grad_t, double *work, hipStream_t s);
+                                   double *grad_t, void *work, hipStream_t s);
 
 // ---- ALL_NEIGHBORS list conversions --------------------------------------
 // prior matches (scamp_hip_match[n * k], corr > -2 = a match) -> key lists

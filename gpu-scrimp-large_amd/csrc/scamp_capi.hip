@@ -897,6 +897,9 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
   const int64_t m = window;
   if (m < 3 || (int64_t)n < m || batch < 1)
     return fail(SCAMP_HIP_INVALID_ARGS, "invalid window, length or batch");
+  if (batch > (int64_t)0x7FFFFFFF / ((int64_t)n - m + 1))
+    return fail(SCAMP_HIP_INVALID_ARGS,
+                "backward: at most 2^31 - 1 profile entries per call");
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
@@ -912,8 +915,9 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
     double *dg = ar.alloc<double>(batch * stat_len);
     HIP_CHECK(launch_stats(d_T, (int64_t)n, m, clean, clean_len, mean, inorm,
                            df, dg, stat_len, s, batch, (int64_t)n));
-    double *work = ar.alloc<double>(2 * batch * ns);
-    HIP_CHECK(hipMemsetAsync(d_grad_T, 0, sizeof(double) * n * batch, s));
+    const size_t work_bytes = pearson_backward_workspace((int64_t)n, m, batch);
+    if (work_bytes == 0) throw HipFailure{hipErrorInvalidValue, "backward workspace"};
+    char *work = ar.alloc<char>((int64_t)work_bytes);
     HIP_CHECK(launch_pearson_backward(d_T, (int64_t)n, m, batch, mean, inorm,
                                       stat_len, d_index, d_grad_corr, d_grad_T,
                                       work, s));
