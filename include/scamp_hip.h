@@ -166,7 +166,10 @@ int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
  * and the forward's 1NN indices d_index[batch][n-m+1] (int32, -1 = none),
  *   d_grad_T[b][t] = sum_i d_grad_corr[b][i] * d corr(T[i:i+m], T[I_i:I_i+m])
  *                                             / d T[t]
- * with the index held fixed.  d_grad_T is overwritten.  Blocks until done. */
+ * with the index held fixed.  d_grad_T is overwritten; the result does not
+ * depend on scheduling (no atomics: the same bits on every call).  At most
+ * 2^31 - 1 profile entries (batch * (n - m + 1)) per call.  Blocks until
+ * done. */
 int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
                                int64_t window, int64_t batch,
                                const int32_t *d_index,
