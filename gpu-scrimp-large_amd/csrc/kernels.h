@@ -141,8 +141,11 @@ struct SweepArgs {
 };
 
 // cols/rows: which profiles the pass updates (for kMatrix: direct cells
-// (row, col) and transposed cells (col, row) respectively).
+// (row, col) and transposed cells (col, row) respectively).  fresh (kNN):
+// the launch's thresholds start far from final (short series cut into row
+// chunks); selects the kernel variant that makes the merge path cheaper.
 hipError_t launch_sweep(const SweepArgs &a, bool fp32, SweepKind kind,
-                        bool cols, bool rows, int grid, hipStream_t s);
+                        bool cols, bool rows, int grid, hipStream_t s,
+                        bool fresh);
 
 }  // namespace scamp_amd

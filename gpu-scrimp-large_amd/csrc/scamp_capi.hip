@@ -339,7 +339,7 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   const int grid = (int)std::min<int64_t>(nitems, (int64_t)ctx->num_cus * 16);
   Event e0, e1;
   HIP_CHECK(hipEventRecord(e0.e, s));
-  HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s));
+  HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, chunked));
   HIP_CHECK(hipEventRecord(e1.e, s));
   HIP_CHECK(hipEventSynchronize(e1.e));
   float ms = 0;

@@ -99,7 +99,13 @@ struct CellAcc {
   int64_t pos;         // (first column of the lane) mod cells-per-step
 };
 
-template <typename DT, int KIND, bool COLS, bool ROWS>
+// FRESH (1NN only): a launch whose profile thresholds start far from their
+// final values (short series cut into row chunks), where many rows take the
+// merge path.  Its rows merge into an LDS table flushed once per block (no
+// wave-wide reduction per row) and run only the side of the merge whose test
+// passed.  Kept out of the long-series kernel, whose hot loop it would
+// perturb.
+template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH = false>
 struct Sweep {
   const SweepArgs &a;
   const int lane;
@@ -127,6 +133,8 @@ struct Sweep {
   DT cov[K];
   DT hi[K];
   uint64_t *s_kq = nullptr;  // ALL_NEIGHBORS: per-lane candidate slots (LDS)
+  static constexpr bool kTable = KIND == kNN && FRESH;
+  uint64_t *s_rt = nullptr;  // FRESH: the block's row table (LDS, kRows keys)
   // SUM_THRESH rows: this lane's partial sum of each row of the current block
   double rsum[K];
   Slot<DT> S[K];         // column slots, indexed by column mod K
@@ -259,6 +267,7 @@ struct Sweep {
       }
     }
     __builtin_amdgcn_wave_barrier();
+    if (kTable && lane < kRows) s_rt[lane] = 0;  // the FP32 table shares s_ay
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
@@ -426,8 +435,10 @@ struct Sweep {
   }
 
   // -------------------------------------------------- 1NN slow path
+  // Row rr of the block (row i): the wave's best pair into the row's entry
+  // (FRESH: each lane's best into the block's LDS row table instead).
   __device__ __forceinline__ void nn_row_update(const DT (&c)[K], int64_t i,
-                                                DT trow) {
+                                                int rr, DT trow) {
     float bv = -3.0f;
     int bk = -1;
 #pragma unroll
@@ -441,10 +452,50 @@ struct Sweep {
     uint64_t key = 0;
     if (bk >= 0 && (DT)bv >= trow)
       key = make_key(bv, static_cast<uint32_t>(i + d0 + bk));
+    if (kTable) {
+      if (key != 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(s_rt + rr),
+                  static_cast<unsigned long long>(key));
+      return;
+    }
     key = wave_max_u64(key);
     if (lane == 0 && key != 0)
       atomicMax(reinterpret_cast<unsigned long long *>(a.keyY + i),
                 static_cast<unsigned long long>(key));
+  }
+  // Row i's pairs that beat their column's best, into the columns' entries.
+  __device__ __forceinline__ void nn_col_update(int r, const DT (&c)[K],
+                                                int64_t i) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      Slot<DT> &s = S[(r + k) % K];
+      if (c[k] >= s.t) {
+        const float f = (float)c[k];
+        const uint64_t key = make_key(f, static_cast<uint32_t>(i));
+        atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + i + d0 + k),
+                  static_cast<unsigned long long>(key));
+        // the column's best is now at least f (no need to wait for the
+        // atomic's return value); the tighter threshold travels with the
+        // column to the next lanes
+        const DT tf = Thresh<DT>::from_key_hi(enc_corr(f));
+        s.t = s.t > tf ? s.t : tf;
+      }
+    }
+  }
+
+  // FRESH: the block's row table (rows i_b + l, l < kRows) into the profile,
+  // and cleared for the next block.
+  __device__ __forceinline__ void flush_row_table(int64_t i_b) {
+    __builtin_amdgcn_wave_barrier();
+    if (lane < kRows) {
+      const uint64_t v = s_rt[lane];
+      if (v != 0) {
+        atomicMax(reinterpret_cast<unsigned long long *>(a.keyY + i_b + lane),
+                  static_cast<unsigned long long>(v));
+        s_rt[lane] = 0;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 
   // ------------------------------------------------- ALL_NEIGHBORS lists
@@ -567,7 +618,18 @@ struct Sweep {
       c[k] = cv * S[(r + k) % K].in * rin;
       if (MASKED && i + d0 + k >= a.nX) c[k] = nan_of<DT>();
     }
-    if (kThr) {
+    if (kThr && kTable) {
+      // FRESH: the row and column masks are kept apart, so that a row where
+      // only one side passes runs only that side's merge.
+      uint64_t hit_c = 0, hit_r = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (COLS) hit_c |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].t);
+        if (ROWS) hit_r |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
+      }
+      if (ROWS && hit_r) nn_row_update(c, i, rr, trow);
+      if (COLS && hit_c) nn_col_update(r, c, i);
+    } else if (kThr) {
       // One compare per test written straight into a wave mask (keeps the
       // compiler from merging the two tests into a NaN-canonicalising min).
       uint64_t hit = 0;
@@ -579,25 +641,8 @@ struct Sweep {
       if (KIND == kKnn && hit) {
         knn_update(r, c, i, trow);
       } else if (hit) {
-        if (ROWS) nn_row_update(c, i, trow);
-        if (COLS) {
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            Slot<DT> &s = S[(r + k) % K];
-            if (c[k] >= s.t) {
-              const float f = (float)c[k];
-              const uint64_t key = make_key(f, static_cast<uint32_t>(i));
-              atomicMax(
-                  reinterpret_cast<unsigned long long *>(a.keyX + i + d0 + k),
-                  static_cast<unsigned long long>(key));
-              // the column's best is now at least f (no need to wait for the
-              // atomic's return value); the tighter threshold travels with
-              // the column to the next lanes
-              const DT tf = Thresh<DT>::from_key_hi(enc_corr(f));
-              s.t = s.t > tf ? s.t : tf;
-            }
-          }
-        }
+        if (ROWS) nn_row_update(c, i, rr, trow);
+        if (COLS) nn_col_update(r, c, i);
       }
     } else if (KIND == kSum) {
       // a row with no correlation above the threshold anywhere in the wave
@@ -749,6 +794,7 @@ struct Sweep {
       if (kComp && rr % kRenormRows == kRenormRows - 1) renormalise();
     }
     if (KIND == kSum && ROWS) flush_row_sums(i_b, re - i_b);
+    if (kTable && ROWS) flush_row_table(i_b);
     if (KIND == kMatrix && FAST) cells_fast_end();
     if (ended) return true;
     __builtin_amdgcn_wave_barrier();
@@ -807,7 +853,7 @@ constexpr int sweep_waves_per_eu() {
   return 1;
 }
 
-template <typename DT, int KIND, bool COLS, bool ROWS>
+template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH>
 __global__ __launch_bounds__(kWave)
 __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<DT, KIND>())))
 void sweep_kernel(SweepArgs a) {
@@ -817,7 +863,15 @@ void sweep_kernel(SweepArgs a) {
   __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
   __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
   __shared__ uint64_t s_kq[KIND == kKnn ? kBand : 1];
+  // FRESH: best key of each row of the current block (zero = none).  The
+  // FP32 form keeps it in the anchor staging, which is only in use between
+  // blocks (anchor() clears the table again when it is done).
+  constexpr bool kTable = KIND == kNN && FRESH;
+  __shared__ uint64_t s_rt_own[kTable && sizeof(DT) == 8 ? 2 * kDiagsPerLane : 1];
+  uint64_t *const s_rt =
+      sizeof(DT) == 4 ? reinterpret_cast<uint64_t *>(s_ay) : s_rt_own;
   const int lane = threadIdx.x;
+  if (kTable && lane < 2 * kDiagsPerLane) s_rt[lane] = 0;
   const unsigned int nitems = (unsigned int)a.nitems;
   for (;;) {
     unsigned int b = 0;
@@ -850,8 +904,9 @@ void sweep_kernel(SweepArgs a) {
       la.keyY = a.keyY + o;
     }
     const int64_t D0 = la.d_base + (int64_t)band * kBand;
-    Sweep<DT, KIND, COLS, ROWS> sw(la, lane, D0);
+    Sweep<DT, KIND, COLS, ROWS, FRESH> sw(la, lane, D0);
     sw.s_kq = s_kq;
+    if (kTable) sw.s_rt = s_rt;
     for (int seg = 0; seg < 2; ++seg) {
       const int64_t rs = sizeof(DT) == 4 ? (seg == 0 ? iv.rs : iv.rs2)
                                          : (seg == 0 ? it->rs : it->rs2);
@@ -870,17 +925,17 @@ void sweep_kernel(SweepArgs a) {
   }
 }
 
-template <typename DT, int KIND>
+template <typename DT, int KIND, bool FRESH = false>
 hipError_t launch_kind(const SweepArgs &a, bool cols, bool rows, int grid,
                        hipStream_t s) {
   if (cols && rows) {
-    hipLaunchKernelGGL((sweep_kernel<DT, KIND, true, true>), dim3(grid),
+    hipLaunchKernelGGL((sweep_kernel<DT, KIND, true, true, FRESH>), dim3(grid),
                        dim3(kWave), 0, s, a);
   } else if (cols) {
-    hipLaunchKernelGGL((sweep_kernel<DT, KIND, true, false>), dim3(grid),
+    hipLaunchKernelGGL((sweep_kernel<DT, KIND, true, false, FRESH>), dim3(grid),
                        dim3(kWave), 0, s, a);
   } else if (rows) {
-    hipLaunchKernelGGL((sweep_kernel<DT, KIND, false, true>), dim3(grid),
+    hipLaunchKernelGGL((sweep_kernel<DT, KIND, false, true, FRESH>), dim3(grid),
                        dim3(kWave), 0, s, a);
   }
   return hipGetLastError();
