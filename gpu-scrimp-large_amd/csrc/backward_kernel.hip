@@ -12,11 +12,11 @@
 //
 // Every sample's gradient is gathered by one thread in a fixed order, with no
 // atomics, so the result is the same bit for bit from run to run:
-//   1. one wavefront per profile entry: r and three coefficients;
+//   1. one thread per profile entry: r and three coefficients;
 //   2. the entries sorted by partner window (radix sort of (partner, entry));
 //   3. per sample t, the self half over the windows i that contain t, then
 //      the partner half over the entries whose partner window contains t.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "device_common.h"
 #include "kernels.h"
@@ -25,7 +25,6 @@ namespace scamp_amd {
 namespace {
 
 constexpr int kBwdBlock = 256;
-constexpr int kBwdWaves = kBwdBlock / kWave;
 constexpr int kBwdChunk = 512;  // entries staged through LDS at a time
 
 // Per entry e = b * ns + i (partner j = I_i, g = dL/dP_i):
@@ -38,19 +37,21 @@ struct Coef {
   double P, Q, R;
 };
 
-// Pass 1, one wavefront per (series b, entry i).  Means and inverse norms
-// come from the statistics kernels (one row of `stat_stride` entries per
-// series).
+// Pass 1, one thread per (series b, entry i).  Means and inverse norms come
+// from the statistics kernels (one row of `stat_stride` entries per series).
+// The windows i of a wavefront's lanes overlap, so their loads coalesce; each
+// lane's partner window streams through the cache.  (A wavefront per entry
+// spreads the m terms over the lanes, but launches one wavefront for every
+// profile entry and is bound by wave dispatch.)
 __global__ __launch_bounds__(kBwdBlock) void bwd_coef_kernel(
     const double *__restrict__ T, int64_t n, int64_t m, int64_t ns,
     int64_t batch, const double *__restrict__ mean,
     const double *__restrict__ inorm, int64_t stat_stride,
     const int32_t *__restrict__ index, const double *__restrict__ grad_p,
-    Coef *__restrict__ coef, uint64_t *__restrict__ key,
-    uint64_t *__restrict__ val) {
-  const int lane = threadIdx.x & (kWave - 1);
+    Coef *__restrict__ coef, uint32_t *__restrict__ key,
+    uint32_t *__restrict__ val) {
   const int64_t b = blockIdx.y;
-  const int64_t i = (int64_t)blockIdx.x * kBwdWaves + threadIdx.x / kWave;
+  const int64_t i = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
   if (i >= ns) return;
   const int64_t e = b * ns + i;
   const int64_t j = index[e];
@@ -58,24 +59,28 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_coef_kernel(
   const double ni = inorm[b * stat_stride + i];
   const double nj = j >= 0 && j < ns ? inorm[b * stat_stride + j] : nan("");
   if (j < 0 || j >= ns || g == 0.0 || !(ni == ni) || !(nj == nj)) {
-    if (lane == 0) {
-      coef[e] = Coef{0.0, 0.0, 0.0};
-      key[e] = (uint64_t)(batch * ns);  // after every real partner
-      val[e] = (uint64_t)e;
-    }
+    coef[e] = Coef{0.0, 0.0, 0.0};
+    key[e] = (uint32_t)(batch * ns);  // after every real partner
+    val[e] = (uint32_t)e;
     return;
   }
-  const double *Tb = T + b * n;
+  const double *x = T + b * n + i;
+  const double *y = T + b * n + j;
   const double mi = mean[b * stat_stride + i], mj = mean[b * stat_stride + j];
-  double part = 0.0;
-  for (int64_t k = lane; k < m; k += kWave)
-    part = fma((Tb[i + k] - mi) * ni, (Tb[j + k] - mj) * nj, part);
-  const double r = wave_sum(part);
-  if (lane == 0) {
-    coef[e] = Coef{g * ni * nj, g * r * ni * ni, g * r * nj * nj};
-    key[e] = (uint64_t)(b * ns + j);
-    val[e] = (uint64_t)e;
+  // four independent chains keep four loads of each window in flight
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t k = 0;
+  for (; k + 4 <= m; k += 4) {
+    s0 = fma(x[k] - mi, y[k] - mj, s0);
+    s1 = fma(x[k + 1] - mi, y[k + 1] - mj, s1);
+    s2 = fma(x[k + 2] - mi, y[k + 2] - mj, s2);
+    s3 = fma(x[k + 3] - mi, y[k + 3] - mj, s3);
   }
+  for (; k < m; ++k) s0 = fma(x[k] - mi, y[k] - mj, s0);
+  const double r = ((s0 + s1) + (s2 + s3)) * ni * nj;
+  coef[e] = Coef{g * ni * nj, g * r * ni * ni, g * r * nj * nj};
+  key[e] = (uint32_t)(b * ns + j);
+  val[e] = (uint32_t)e;
 }
 
 // Pass 3a, one thread per sample t: the self half, a gather over the windows
@@ -129,11 +134,24 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_self_kernel(
 }
 
 // First position in keys[0, len) holding a value >= v.
-__device__ int64_t lower_bound_u64(const uint64_t *keys, int64_t len, uint64_t v) {
+__device__ int64_t lower_bound_u32(const uint32_t *keys, int64_t len, uint32_t v) {
   int64_t lo = 0, hi = len;
   while (lo < hi) {
     const int64_t mid = (lo + hi) / 2;
     if (keys[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// First position in v[0, len) (ascending) holding a value >= x.
+__device__ __forceinline__ int first_at_least(const int32_t *v, int len, int64_t x) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (v[mid] < x)
       lo = mid + 1;
     else
       hi = mid;
@@ -147,8 +165,8 @@ __device__ int64_t lower_bound_u64(const uint64_t *keys, int64_t len, uint64_t v
 __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
     const double *__restrict__ T, int64_t n, int64_t m, int64_t ns,
     const double *__restrict__ mean, int64_t stat_stride,
-    const Coef *__restrict__ coef, const uint64_t *__restrict__ skey,
-    const uint64_t *__restrict__ sval, int64_t nent,
+    const Coef *__restrict__ coef, const uint32_t *__restrict__ skey,
+    const uint32_t *__restrict__ sval, int64_t nent,
     double *__restrict__ grad_t) {
   __shared__ double sP[kBwdChunk], sR[kBwdChunk], sMi[kBwdChunk], sMj[kBwdChunk];
   __shared__ int32_t sI[kBwdChunk], sJ[kBwdChunk];
@@ -162,8 +180,8 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
   const int64_t jlo = t0 - m + 1 > 0 ? t0 - m + 1 : 0;
   const int64_t jhi = t0 + kBwdBlock - 1 < ns - 1 ? t0 + kBwdBlock - 1 : ns - 1;
   if (threadIdx.x == 0) {
-    range[0] = lower_bound_u64(skey, nent, (uint64_t)(b * ns + jlo));
-    range[1] = lower_bound_u64(skey, nent, (uint64_t)(b * ns + jhi + 1));
+    range[0] = lower_bound_u32(skey, nent, (uint32_t)(b * ns + jlo));
+    range[1] = lower_bound_u32(skey, nent, (uint32_t)(b * ns + jhi + 1));
   }
   __syncthreads();
   const int64_t p0 = range[0], p1 = range[1];
@@ -185,14 +203,12 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
     }
     __syncthreads();
     if (t < n) {
-      // the chunk's entries with partner j in [t-m+1, t] (sJ is sorted)
-      int a = 0, z = cn;
-      const int64_t want_lo = t - m + 1;
-      while (a < z) {
-        const int mid = (a + z) / 2;
-        if (sJ[mid] < want_lo) a = mid + 1; else z = mid;
-      }
-      for (int q = a; q < cn && sJ[q] <= t; ++q) {
+      // the chunk's entries with partner j in [t-m+1, t] (sJ is sorted); the
+      // count is known before the loop, so its loads can be issued together
+      const int a = first_at_least(sJ, cn, t - m + 1);
+      const int z = first_at_least(sJ, cn, t + 1);
+#pragma unroll 4
+      for (int q = a; q < z; ++q) {
         const int64_t i = sI[q], j = sJ[q];
         acc += sP[q] * (Tb[i + t - j] - sMi[q]) - sR[q] * (tt - sMj[q]);
       }
@@ -205,23 +221,37 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
 // temporary storage.
 struct BwdWork {
   Coef *coef;
-  uint64_t *key_in, *key_out, *val_in, *val_out;
+  uint32_t *key_in, *key_out, *val_in, *val_out;
   void *tmp;
   size_t tmp_bytes;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+// The sort is stable (radix), so entries of one partner keep entry order.
+// Merge sort is rocPRIM's default below 2^20 items; Onesweep (a few digit
+// passes) is several times faster at the sizes of a batch of profiles, so
+// the merge-sort limit is set to 0.
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config,
+                                              rocprim::default_config,
+                                              rocprim::default_config, 0>;
+
+hipError_t sort_pairs(void *tmp, size_t &bytes, const uint32_t *kin, uint32_t *kout,
+                      const uint32_t *vin, uint32_t *vout, int64_t nent,
+                      int end_bit, hipStream_t s) {
+  return rocprim::radix_sort_pairs<SortConfig>(tmp, bytes, kin, kout, vin, vout,
+                                               static_cast<size_t>(nent), 0u,
+                                               static_cast<unsigned>(end_bit), s);
+}
+
 hipError_t sort_bytes(int64_t nent, int end_bit, size_t *bytes) {
-  return hipcub::DeviceRadixSort::SortPairs(
-      nullptr, *bytes, static_cast<const uint64_t *>(nullptr),
-      static_cast<uint64_t *>(nullptr), static_cast<const uint64_t *>(nullptr),
-      static_cast<uint64_t *>(nullptr), static_cast<int>(nent), 0, end_bit);
+  return sort_pairs(nullptr, *bytes, nullptr, nullptr, nullptr, nullptr, nent,
+                    end_bit, nullptr);
 }
 
 int key_bits(int64_t nent) {
   int bits = 1;
-  while (bits < 63 && (int64_t(1) << bits) <= nent) ++bits;  // keys <= nent
+  while (bits < 32 && (int64_t(1) << bits) <= nent) ++bits;  // keys <= nent
   return bits;
 }
 
@@ -230,14 +260,14 @@ BwdWork carve(void *base, int64_t nent, size_t tmp_bytes) {
   BwdWork w;
   w.coef = reinterpret_cast<Coef *>(p);
   p += align256(sizeof(Coef) * nent);
-  w.key_in = reinterpret_cast<uint64_t *>(p);
-  p += align256(8 * nent);
-  w.key_out = reinterpret_cast<uint64_t *>(p);
-  p += align256(8 * nent);
-  w.val_in = reinterpret_cast<uint64_t *>(p);
-  p += align256(8 * nent);
-  w.val_out = reinterpret_cast<uint64_t *>(p);
-  p += align256(8 * nent);
+  w.key_in = reinterpret_cast<uint32_t *>(p);
+  p += align256(4 * nent);
+  w.key_out = reinterpret_cast<uint32_t *>(p);
+  p += align256(4 * nent);
+  w.val_in = reinterpret_cast<uint32_t *>(p);
+  p += align256(4 * nent);
+  w.val_out = reinterpret_cast<uint32_t *>(p);
+  p += align256(4 * nent);
   w.tmp = p;
   w.tmp_bytes = tmp_bytes;
   return w;
@@ -250,7 +280,7 @@ size_t pearson_backward_workspace(int64_t n, int64_t m, int64_t batch) {
   if (nent <= 0) return 0;
   size_t tmp = 0;
   if (sort_bytes(nent, key_bits(nent), &tmp) != hipSuccess) return 0;
-  return align256(sizeof(Coef) * nent) + 4 * align256(8 * nent) + align256(tmp);
+  return align256(sizeof(Coef) * nent) + 4 * align256(4 * nent) + align256(tmp);
 }
 
 hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
@@ -268,13 +298,12 @@ hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
   BwdWork w = carve(work, nent, tmp);
   const unsigned nb = static_cast<unsigned>(batch);
   hipLaunchKernelGGL(bwd_coef_kernel,
-                     dim3(static_cast<unsigned>((ns + kBwdWaves - 1) / kBwdWaves), nb),
+                     dim3(static_cast<unsigned>((ns + kBwdBlock - 1) / kBwdBlock), nb),
                      dim3(kBwdBlock), 0, s, T, n, m, ns, batch, mean, inorm,
                      stat_stride, index, grad_p, w.coef, w.key_in, w.val_in);
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  err = hipcub::DeviceRadixSort::SortPairs(w.tmp, w.tmp_bytes, w.key_in, w.key_out,
-                                           w.val_in, w.val_out, static_cast<int>(nent),
-                                           0, bits, s);
+  err = sort_pairs(w.tmp, w.tmp_bytes, w.key_in, w.key_out, w.val_in, w.val_out,
+                   nent, bits, s);
   if (err != hipSuccess) return err;
   const dim3 grid(static_cast<unsigned>((n + kBwdBlock - 1) / kBwdBlock), nb);
   hipLaunchKernelGGL(bwd_self_kernel, grid, dim3(kBwdBlock), 0, s, T, n, m, ns,

@@ -194,6 +194,9 @@ struct Pass {
   uint64_t *knnX, *knnY;  // ALL_NEIGHBORS lists
 };
 
+// Row chunks up to this length run the 1NN sweep's short-chunk variant.
+constexpr int64_t kFreshMaxRows = 16384;
+
 // Runs one pass (one diagonal range of one orientation).
 void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
               bool fp32, SweepKind kind, uint32_t *mat, int64_t mat_w,
@@ -339,7 +342,11 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   const int grid = (int)std::min<int64_t>(nitems, (int64_t)ctx->num_cus * 16);
   Event e0, e1;
   HIP_CHECK(hipEventRecord(e0.e, s));
-  HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, chunked));
+  // Short row chunks start with the profile far below its final values, so
+  // many of their rows merge: the 1NN sweep then runs its variant with
+  // block-level merge tables (the long-chunk kernel keeps fewer registers).
+  const bool fresh = chunked && R <= kFreshMaxRows;
+  HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, fresh));
   HIP_CHECK(hipEventRecord(e1.e, s));
   HIP_CHECK(hipEventSynchronize(e1.e));
   float ms = 0;

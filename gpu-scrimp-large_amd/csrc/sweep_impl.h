@@ -100,11 +100,12 @@ struct CellAcc {
 };
 
 // FRESH (1NN only): a launch whose profile thresholds start far from their
-// final values (short series cut into row chunks), where many rows take the
-// merge path.  Its rows merge into an LDS table flushed once per block (no
-// wave-wide reduction per row) and run only the side of the merge whose test
-// passed.  Kept out of the long-series kernel, whose hot loop it would
-// perturb.
+// final values (short series cut into short row chunks), where many rows take
+// the merge path.  Its rows merge into an LDS table flushed once per block (no
+// wave-wide reduction per row), in FP64 its columns into a second one (no
+// global atomic per record), and a row runs only the side of the merge whose
+// test passed.  Kept out of the long-chunk kernel, whose hot loop the extra
+// registers would slow down.
 template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH = false>
 struct Sweep {
   const SweepArgs &a;
@@ -135,6 +136,13 @@ struct Sweep {
   uint64_t *s_kq = nullptr;  // ALL_NEIGHBORS: per-lane candidate slots (LDS)
   static constexpr bool kTable = KIND == kNN && FRESH;
   uint64_t *s_rt = nullptr;  // FRESH: the block's row table (LDS, kRows keys)
+  // FRESH, FP64: the block's column table (LDS).  Column i_b + D0 + p, p in
+  // [0, kBand + kRows - 1), sits at (p % K) * kColStride + p / K, so the
+  // lanes of one merge (p = rr + k + K * lane) hit consecutive entries.
+  static constexpr bool kColTable = kTable && COLS && sizeof(DT) == 8;
+  static constexpr int kColStride = (kBand + kRows - 1 + K - 1) / K;
+  uint64_t *s_ct = nullptr;
+  bool ct_dirty = false;     // wave-uniform: some column merged this block
   // SUM_THRESH rows: this lane's partial sum of each row of the current block
   double rsum[K];
   Slot<DT> S[K];         // column slots, indexed by column mod K
@@ -463,17 +471,24 @@ struct Sweep {
       atomicMax(reinterpret_cast<unsigned long long *>(a.keyY + i),
                 static_cast<unsigned long long>(key));
   }
-  // Row i's pairs that beat their column's best, into the columns' entries.
-  __device__ __forceinline__ void nn_col_update(int r, const DT (&c)[K],
+  // Row i's pairs that beat their column's best, into the columns' entries
+  // (FRESH, FP64: into the block's column table).
+  __device__ __forceinline__ void nn_col_update(int r, int rr, const DT (&c)[K],
                                                 int64_t i) {
+    if (kColTable) ct_dirty = true;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       Slot<DT> &s = S[(r + k) % K];
       if (c[k] >= s.t) {
         const float f = (float)c[k];
         const uint64_t key = make_key(f, static_cast<uint32_t>(i));
-        atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + i + d0 + k),
-                  static_cast<unsigned long long>(key));
+        if (kColTable)
+          atomicMax(reinterpret_cast<unsigned long long *>(
+                        s_ct + ((rr + k) % K) * kColStride + (rr + k) / K + lane),
+                    static_cast<unsigned long long>(key));
+        else
+          atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + i + d0 + k),
+                    static_cast<unsigned long long>(key));
         // the column's best is now at least f (no need to wait for the
         // atomic's return value); the tighter threshold travels with the
         // column to the next lanes
@@ -495,6 +510,22 @@ struct Sweep {
         s_rt[lane] = 0;
       }
     }
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  // FRESH, FP64: the block's column table into the profile, and cleared.
+  __device__ __forceinline__ void flush_col_table(int64_t i_b) {
+    __builtin_amdgcn_wave_barrier();
+    for (int p = lane; p < kBand + kRows - 1; p += kWave) {
+      uint64_t *e = s_ct + (p % K) * kColStride + p / K;
+      const uint64_t v = *e;
+      if (v != 0) {
+        atomicMax(reinterpret_cast<unsigned long long *>(a.keyX + i_b + D0 + p),
+                  static_cast<unsigned long long>(v));
+        *e = 0;
+      }
+    }
+    ct_dirty = false;
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -628,7 +659,7 @@ struct Sweep {
         if (ROWS) hit_r |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
       }
       if (ROWS && hit_r) nn_row_update(c, i, rr, trow);
-      if (COLS && hit_c) nn_col_update(r, c, i);
+      if (COLS && hit_c) nn_col_update(r, rr, c, i);
     } else if (kThr) {
       // One compare per test written straight into a wave mask (keeps the
       // compiler from merging the two tests into a NaN-canonicalising min).
@@ -642,7 +673,7 @@ struct Sweep {
         knn_update(r, c, i, trow);
       } else if (hit) {
         if (ROWS) nn_row_update(c, i, rr, trow);
-        if (COLS) nn_col_update(r, c, i);
+        if (COLS) nn_col_update(r, rr, c, i);
       }
     } else if (KIND == kSum) {
       // a row with no correlation above the threshold anywhere in the wave
@@ -795,6 +826,7 @@ struct Sweep {
     }
     if (KIND == kSum && ROWS) flush_row_sums(i_b, re - i_b);
     if (kTable && ROWS) flush_row_table(i_b);
+    if (kColTable && ct_dirty) flush_col_table(i_b);
     if (KIND == kMatrix && FAST) cells_fast_end();
     if (ended) return true;
     __builtin_amdgcn_wave_barrier();
@@ -868,10 +900,14 @@ void sweep_kernel(SweepArgs a) {
   // blocks (anchor() clears the table again when it is done).
   constexpr bool kTable = KIND == kNN && FRESH;
   __shared__ uint64_t s_rt_own[kTable && sizeof(DT) == 8 ? 2 * kDiagsPerLane : 1];
+  using SW = Sweep<DT, KIND, COLS, ROWS, FRESH>;
+  __shared__ uint64_t s_ct[SW::kColTable ? kDiagsPerLane * SW::kColStride : 1];
   uint64_t *const s_rt =
       sizeof(DT) == 4 ? reinterpret_cast<uint64_t *>(s_ay) : s_rt_own;
   const int lane = threadIdx.x;
   if (kTable && lane < 2 * kDiagsPerLane) s_rt[lane] = 0;
+  if (SW::kColTable)
+    for (int q = lane; q < kDiagsPerLane * SW::kColStride; q += kWave) s_ct[q] = 0;
   const unsigned int nitems = (unsigned int)a.nitems;
   for (;;) {
     unsigned int b = 0;
@@ -907,6 +943,7 @@ void sweep_kernel(SweepArgs a) {
     Sweep<DT, KIND, COLS, ROWS, FRESH> sw(la, lane, D0);
     sw.s_kq = s_kq;
     if (kTable) sw.s_rt = s_rt;
+    if (SW::kColTable) sw.s_ct = s_ct;
     for (int seg = 0; seg < 2; ++seg) {
       const int64_t rs = sizeof(DT) == 4 ? (seg == 0 ? iv.rs : iv.rs2)
                                          : (seg == 0 ? it->rs : it->rs2);

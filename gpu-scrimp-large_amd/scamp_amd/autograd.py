@@ -24,17 +24,19 @@ import torch
 from . import _capi as C
 
 _INIT = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
-_LOW = 0xFFFFFFFF
 
 
 def _forward(t64: torch.Tensor, m: int, device: int):
     B, n = t64.shape
-    prof = torch.full((B, n - m + 1), _INIT, dtype=torch.int64, device=t64.device)
+    ns = n - m + 1
+    prof = torch.full((B, ns), _INIT, dtype=torch.int64, device=t64.device)
     torch.cuda.synchronize(t64.device)
     C.selfjoin_batch_device(t64.data_ptr(), n, B, C.make_params(m, precision="double"),
                             prof.data_ptr(), device=device)
-    corr = (prof & _LOW).to(torch.int32).view(torch.float32)
-    idx = (prof >> 32).to(torch.int32)
+    # mp_entry: the float32 correlation in the low word, the int32 index in
+    # the high word (views, no copies)
+    corr = prof.view(torch.float32).view(B, ns, 2)[..., 0]
+    idx = prof.view(torch.int32).view(B, ns, 2)[..., 1]
     return corr, idx
 
 
@@ -49,8 +51,9 @@ class MatrixProfile(torch.autograd.Function):
         device = T.device.index or 0
         t64 = T.detach().to(torch.float64).contiguous()
         P, I = _forward(t64, m, device)
-        I = torch.where(P < -1, torch.full_like(I, -1), I)
-        P = torch.where(P < -1, torch.full_like(P, float("nan")), P)
+        empty = P < -1
+        I = torch.where(empty, -1, I)
+        P = torch.where(empty, float("nan"), P)
         ctx.save_for_backward(t64, I)
         ctx.m = m
         ctx.device = device
