@@ -102,9 +102,9 @@ struct CellAcc {
 // FRESH (1NN only): a launch whose profile thresholds start far from their
 // final values (short series cut into short row chunks), where many rows take
 // the merge path.  Its rows merge into an LDS table flushed once per block (no
-// wave-wide reduction per row), in FP64 its columns into a second one (no
-// global atomic per record), and a row runs only the side of the merge whose
-// test passed.  Kept out of the long-chunk kernel, whose hot loop the extra
+// wave-wide reduction per row), its columns into a second one (no global
+// atomic per record), and a row runs only the side of the merge whose test
+// passed.  Kept out of the long-chunk kernel, whose hot loop the extra
 // registers would slow down.
 template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH = false>
 struct Sweep {
@@ -136,10 +136,10 @@ struct Sweep {
   uint64_t *s_kq = nullptr;  // ALL_NEIGHBORS: per-lane candidate slots (LDS)
   static constexpr bool kTable = KIND == kNN && FRESH;
   uint64_t *s_rt = nullptr;  // FRESH: the block's row table (LDS, kRows keys)
-  // FRESH, FP64: the block's column table (LDS).  Column i_b + D0 + p, p in
+  // FRESH: the block's column table (LDS).  Column i_b + D0 + p, p in
   // [0, kBand + kRows - 1), sits at (p % K) * kColStride + p / K, so the
   // lanes of one merge (p = rr + k + K * lane) hit consecutive entries.
-  static constexpr bool kColTable = kTable && COLS && sizeof(DT) == 8;
+  static constexpr bool kColTable = kTable && COLS;
   static constexpr int kColStride = (kBand + kRows - 1 + K - 1) / K;
   uint64_t *s_ct = nullptr;
   bool ct_dirty = false;     // wave-uniform: some column merged this block
@@ -275,7 +275,10 @@ struct Sweep {
       }
     }
     __builtin_amdgcn_wave_barrier();
-    if (kTable && lane < kRows) s_rt[lane] = 0;  // the FP32 table shares s_ay
+    // the FP32 row and column tables share s_ay and s_ax
+    if (kTable && lane < kRows) s_rt[lane] = 0;
+    if (kColTable)
+      for (int q = lane; q < K * kColStride; q += kWave) s_ct[q] = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
@@ -472,7 +475,7 @@ struct Sweep {
                 static_cast<unsigned long long>(key));
   }
   // Row i's pairs that beat their column's best, into the columns' entries
-  // (FRESH, FP64: into the block's column table).
+  // (FRESH: into the block's column table).
   __device__ __forceinline__ void nn_col_update(int r, int rr, const DT (&c)[K],
                                                 int64_t i) {
     if (kColTable) ct_dirty = true;
@@ -513,7 +516,7 @@ struct Sweep {
     __builtin_amdgcn_wave_barrier();
   }
 
-  // FRESH, FP64: the block's column table into the profile, and cleared.
+  // FRESH: the block's column table into the profile, and cleared.
   __device__ __forceinline__ void flush_col_table(int64_t i_b) {
     __builtin_amdgcn_wave_barrier();
     for (int p = lane; p < kBand + kRows - 1; p += kWave) {
@@ -895,13 +898,20 @@ void sweep_kernel(SweepArgs a) {
   __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
   __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
   __shared__ uint64_t s_kq[KIND == kKnn ? kBand : 1];
-  // FRESH: best key of each row of the current block (zero = none).  The
-  // FP32 form keeps it in the anchor staging, which is only in use between
-  // blocks (anchor() clears the table again when it is done).
+  // FRESH: best key of each row of the current block, and of each column
+  // the block crosses (zero = none).  The FP32 form keeps both tables in the
+  // anchor staging, which is only in use between blocks (anchor() clears
+  // them again when it is done).
   constexpr bool kTable = KIND == kNN && FRESH;
   __shared__ uint64_t s_rt_own[kTable && sizeof(DT) == 8 ? 2 * kDiagsPerLane : 1];
   using SW = Sweep<DT, KIND, COLS, ROWS, FRESH>;
-  __shared__ uint64_t s_ct[SW::kColTable ? kDiagsPerLane * SW::kColStride : 1];
+  __shared__ uint64_t s_ct_own[SW::kColTable && sizeof(DT) == 8
+                                   ? kDiagsPerLane * SW::kColStride
+                                   : 1];
+  static_assert(kDiagsPerLane * SW::kColStride <= kBand + kAnchorChunk,
+                "the FP32 column table fits in the anchor staging");
+  uint64_t *const s_ct =
+      sizeof(DT) == 4 ? reinterpret_cast<uint64_t *>(s_ax) : s_ct_own;
   uint64_t *const s_rt =
       sizeof(DT) == 4 ? reinterpret_cast<uint64_t *>(s_ay) : s_rt_own;
   const int lane = threadIdx.x;
