@@ -64,12 +64,17 @@ struct Thresh;
 template <>
 struct Thresh<double> {
   __device__ static double from_key_hi(uint32_t hi) {
-    return static_cast<double>(dec_corr(hi)) - 0x1p-22;
+    return from_corr(dec_corr(hi));
+  }
+  // the same for a (non-NaN) correlation before its encoding
+  __device__ static double from_corr(float f) {
+    return static_cast<double>(f) - 0x1p-22;
   }
 };
 template <>
 struct Thresh<float> {
   __device__ static float from_key_hi(uint32_t hi) { return dec_corr(hi); }
+  __device__ static float from_corr(float f) { return f; }
 };
 
 __device__ inline uint64_t wave_max_u64(uint64_t v) {

@@ -495,7 +495,7 @@ struct Sweep {
         // the column's best is now at least f (no need to wait for the
         // atomic's return value); the tighter threshold travels with the
         // column to the next lanes
-        const DT tf = Thresh<DT>::from_key_hi(enc_corr(f));
+        const DT tf = Thresh<DT>::from_corr(f);
         s.t = s.t > tf ? s.t : tf;
       }
     }
