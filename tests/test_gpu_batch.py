@@ -112,8 +112,9 @@ def test_batch_argument_errors():
     assert rc == 16 and "one profile" in C.last_error()
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed"])
 @pytest.mark.parametrize("rows,cols", [(False, True), (True, False)])
-def test_batch_honours_row_and_column_flags(rows, cols):
+def test_batch_honours_row_and_column_flags(rows, cols, precision):
     """A self-join computing only columns (or only rows) keeps, for each
     subsequence, its best match on one side: the batch must agree with the
     single joins under the same flags."""
@@ -123,7 +124,8 @@ def test_batch_honours_row_and_column_flags(rows, cols):
     prior = torch.tensor(np.tile(C.init_profile("1NN_INDEX", ns), (B, 1)).view(np.int64),
                          device="cuda")
     T = torch.tensor(X, dtype=torch.float64, device="cuda")
-    params = C.make_params(m, computing_rows=rows, computing_columns=cols)
+    params = C.make_params(m, computing_rows=rows, computing_columns=cols,
+                           precision=precision)
     want = prior.clone()
     torch.cuda.synchronize()
     for b in range(B):
