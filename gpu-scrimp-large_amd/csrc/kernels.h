@@ -68,6 +68,24 @@ hipError_t launch_warm_profile(const double *clean, const double *mean,
                                uint64_t *keyCol, uint64_t *keyRow,
                                hipStream_t s);
 
+// ---- FP64 refinement of mixed 1NN_INDEX profiles (refine_kernel.hip) -----
+// Pairs (u, v) with u an index of the profile's series U and v of V are in
+// the join unless zone > 0 and |a - b + delta| < zone, where a and b are
+// the A and B indices of the pair (u_is_a: u is the A index).  A self-join
+// passes zone = m/4, delta = 0, u_is_a = true.
+struct RefineExclusion {
+  int64_t zone;
+  int64_t delta;
+  bool u_is_a;
+};
+hipError_t launch_refine(uint64_t *key, const uint64_t *prior, int64_t nsU,
+                         const double *TU, const double *muU, const double *inU,
+                         const double *dfU, const double *dgU, int64_t nsV,
+                         const double *TV, const double *muV, const double *inV,
+                         const double *dfV, const double *dgV, int64_t m,
+                         RefineExclusion ex, int64_t batch, int64_t stride,
+                         hipStream_t s);
+
 // ---- backward of the differentiable join (backward_kernel.hip) ------------
 // grad_t[b, :] = sum over profile entries i of grad_p[b, i] * d corr(i, I_i)
 // / d T[b, :] for `batch` series of length n stored back to back; mean/inorm

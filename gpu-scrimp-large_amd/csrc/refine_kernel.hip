@@ -1,0 +1,153 @@
+// FP64 refinement of the mixed form's 1NN_INDEX profiles.
+//
+// The mixed sweep rounds its correlations in FP32 arithmetic, a few units in
+// the last place of an FP32 value.  At the top of a correlation peak the
+// neighbouring lags (j - 1, j, j + 1 for the same window i) are near-ties far
+// below that error, so the mixed form can pick a neighbour of the FP64 form's
+// match (on the benchmark series, 85 % of its index disagreements are one lag
+// away and 94 % at most two).  This pass re-evaluates, in FP64, every entry
+// that the join improved (key different from the caller's prior): the match
+// j and the lags j - 2 .. j + 2, keeping the best under the FP64 form's own rule
+// (highest FP32-rounded correlation, then the lowest index), merged with the
+// prior as the join merges.
+//
+// Cost: an entry whose match continues the previous entry's run (j = previous
+// j + 1, the usual case in a profile) gets its candidates' covariances from
+// the previous ones by the join's own recurrence
+//     cov(u + 1, v + 1) = cov(u, v) + df_V[v] dg_U[u] + df_U[u] dg_V[v]
+// (src/cpu_kernels.cpp:166-171); the first entry of a run and of each
+// thread's chunk computes them directly, m FMAs per candidate.
+// This pass has no counterpart in the reference (its GPU mixed form reports
+// the FP32 values as computed).
+#include "device_common.h"
+#include "kernels.h"
+
+namespace scamp_amd {
+namespace {
+
+constexpr int kRefineChunk = 16;  // consecutive entries per thread
+constexpr int kRefineBlock = 64;
+constexpr int kLags = 2;          // candidates j - kLags .. j + kLags
+constexpr int kCand = 2 * kLags + 1;
+
+__global__ __launch_bounds__(kRefineBlock) void refine_kernel(
+    uint64_t *__restrict__ key, const uint64_t *__restrict__ prior, int64_t nsU,
+    const double *__restrict__ TU, const double *__restrict__ muU,
+    const double *__restrict__ inU, const double *__restrict__ dfU,
+    const double *__restrict__ dgU, int64_t nsV, const double *__restrict__ TV,
+    const double *__restrict__ muV, const double *__restrict__ inV,
+    const double *__restrict__ dfV, const double *__restrict__ dgV, int64_t m,
+    RefineExclusion ex, int64_t stride) {
+  // batched self-joins: series blockIdx.y, every array `stride` entries on
+  const int64_t o = (int64_t)blockIdx.y * stride;
+  key += o;
+  prior += o;
+  TU += o; muU += o; inU += o; dfU += o; dgU += o;
+  TV += o; muV += o; inV += o; dfV += o; dgV += o;
+  const int64_t i0 = ((int64_t)blockIdx.x * kRefineBlock + threadIdx.x) * kRefineChunk;
+  const int64_t i1 = i0 + kRefineChunk < nsU ? i0 + kRefineChunk : nsU;
+  int64_t prev_j = -2;  // match of entry i - 1 when refined, else -2
+  double cov[kCand];
+  bool have[kCand];
+  for (int64_t i = i0; i < i1; ++i) {
+    const uint64_t k = key[i];
+    const uint64_t kp = prior[i];
+    if (k == kp) {  // the prior held: its match is not this join's to refine
+      prev_j = -2;
+      continue;
+    }
+    const int64_t j = key_index(k);
+    const bool run = prev_j >= 0 && prev_j + 1 == j;
+    bool direct = !run;
+    if (run) {
+#pragma unroll
+      for (int q = 0; q < kCand; ++q) {
+        const int64_t v = j - kLags + q;
+        if (v < 0 || v >= nsV) {
+          have[q] = false;
+        } else if (have[q]) {
+          // (i - 1, v - 1) -> (i, v) along their diagonal
+          cov[q] += dfV[v - 1] * dgU[i - 1] + dfU[i - 1] * dgV[v - 1];
+        } else {
+          direct = true;
+        }
+      }
+    }
+    if (direct) {
+      const double mu = muU[i];
+      if (j - kLags >= 0 && j + kLags < nsV) {
+        // all candidates in one pass; the V samples slide through registers
+        // (w[q] = TV[v0 + q + t]), one new load per term
+        const int64_t v0 = j - kLags;
+        double acc[kCand], mv[kCand], w[kCand];
+#pragma unroll
+        for (int q = 0; q < kCand; ++q) {
+          acc[q] = 0.0;
+          mv[q] = muV[v0 + q];
+          w[q] = q + 1 < kCand ? TV[v0 + q] : 0.0;
+        }
+        for (int64_t t = 0; t < m; ++t) {
+          w[kCand - 1] = TV[v0 + kCand - 1 + t];
+          const double x = TU[i + t] - mu;
+#pragma unroll
+          for (int q = 0; q < kCand; ++q) acc[q] = fma(x, w[q] - mv[q], acc[q]);
+#pragma unroll
+          for (int q = 0; q + 1 < kCand; ++q) w[q] = w[q + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < kCand; ++q) {
+          cov[q] = acc[q];
+          have[q] = true;
+        }
+      } else {
+        // near either end of V: the candidates in range one at a time
+#pragma unroll
+        for (int q = 0; q < kCand; ++q) {
+          const int64_t v = j - kLags + q;
+          have[q] = v >= 0 && v < nsV;
+          if (!have[q]) continue;
+          const double mv = muV[v];
+          double s0 = 0.0;
+          for (int64_t t = 0; t < m; ++t) s0 = fma(TU[i + t] - mu, TV[v + t] - mv, s0);
+          cov[q] = s0;
+        }
+      }
+    }
+    uint64_t best = kp;
+    const double iu = inU[i];
+    for (int q = 0; q < kCand; ++q) {
+      const int64_t v = j - kLags + q;
+      if (!have[q]) continue;
+      const double iv = inV[v];
+      if (!(iu == iu) || !(iv == iv)) continue;  // NaN or flat window
+      // the join's exclusion zone (u, v as A / B indices)
+      const int64_t d = ex.u_is_a ? i - v + ex.delta : v - i + ex.delta;
+      if (ex.zone > 0 && (d < ex.zone && d > -ex.zone)) continue;
+      const double corr = cov[q] * iv * iu;
+      const uint64_t kc = make_key(static_cast<float>(corr), static_cast<uint32_t>(v));
+      best = kc > best ? kc : best;
+    }
+    key[i] = best;
+    prev_j = j;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_refine(uint64_t *key, const uint64_t *prior, int64_t nsU,
+                         const double *TU, const double *muU, const double *inU,
+                         const double *dfU, const double *dgU, int64_t nsV,
+                         const double *TV, const double *muV, const double *inV,
+                         const double *dfV, const double *dgV, int64_t m,
+                         RefineExclusion ex, int64_t batch, int64_t stride,
+                         hipStream_t s) {
+  if (nsU <= 0 || batch <= 0) return hipSuccess;
+  const int64_t threads = (nsU + kRefineChunk - 1) / kRefineChunk;
+  const unsigned blocks = static_cast<unsigned>((threads + kRefineBlock - 1) / kRefineBlock);
+  hipLaunchKernelGGL(refine_kernel, dim3(blocks, static_cast<unsigned>(batch)),
+                     dim3(kRefineBlock), 0, s, key, prior, nsU, TU, muU, inU, dfU,
+                     dgU, nsV, TV, muV, inV, dfV, dgV, m, ex, stride);
+  return hipGetLastError();
+}
+
+}  // namespace scamp_amd

@@ -468,6 +468,20 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
                              hipMemcpyDeviceToDevice, s));
     HIP_CHECK(launch_f32_to_enc(reinterpret_cast<float *>(mat), H * W, s));
   }
+  // The mixed form's 1NN_INDEX profiles are refined in FP64 after the sweep
+  // (refine_kernel.hip); entries still equal to the prior are left alone.
+  const bool refine = fp32 && pt == SCAMP_HIP_1NN_INDEX;
+  uint64_t *priorA = nullptr, *priorB = nullptr;
+  if (refine) {
+    priorA = ar.alloc<uint64_t>(nsA);
+    HIP_CHECK(hipMemcpyAsync(priorA, keyA, sizeof(uint64_t) * nsA,
+                             hipMemcpyDeviceToDevice, s));
+    if (sep_rows) {
+      priorB = ar.alloc<uint64_t>(nsB);
+      HIP_CHECK(hipMemcpyAsync(priorB, keyB, sizeof(uint64_t) * nsB,
+                               hipMemcpyDeviceToDevice, s));
+    }
+  }
   HIP_CHECK(hipEventRecord(ev_prep.e, s));
 
   // Passes.  Self-join: diagonals [m/4, nsA) of the upper triangle, each pair
@@ -537,6 +551,24 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   for (const Pass &ps : passes)
     run_pass(ctx, ar, ps, m, fp32, kind, mat, W, rpc, cpc,
              p->distance_threshold, anchor, knn_k, &sweep_ms);
+
+  if (refine) {
+    // profile A holds A's windows matched in B (self: in A); profile B the
+    // reverse.  Excluded pairs as in the passes above.
+    int64_t zone = e, delta = 0;
+    if (!self) {
+      if (p->distributed_start_col >= 0 && p->distributed_start_row >= 0)
+        delta = p->distributed_start_col - p->distributed_start_row;
+      zone = (p->is_aligned != 0 && e > 0) ? e : 0;
+    }
+    HIP_CHECK(launch_refine(keyA, priorA, nsA, A.clean, A.mean, A.inorm, A.df,
+                            A.dg, nsB, B.clean, B.mean, B.inorm, B.df, B.dg, m,
+                            RefineExclusion{zone, delta, true}, 1, 0, s));
+    if (sep_rows)
+      HIP_CHECK(launch_refine(keyB, priorB, nsB, B.clean, B.mean, B.inorm, B.df,
+                              B.dg, nsA, A.clean, A.mean, A.inorm, A.df, A.dg, m,
+                              RefineExclusion{zone, delta, false}, 1, 0, s));
+  }
 
   // Back to SCAMP's layouts.
   if (pt == SCAMP_HIP_1NN_INDEX) {
@@ -619,6 +651,14 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   HIP_CHECK(launch_fill_u64(key, batch * seg, make_key(-2.0f, 0xFFFFFFFFu), s));
   const bool mp = pt == SCAMP_HIP_1NN_INDEX;
   HIP_CHECK(launch_profile_in(d_prof, mp, key, ns, batch, seg, s));
+  // mixed 1NN_INDEX: FP64 refinement after the sweep, as in a single join
+  const bool refine = fp32 && mp;
+  uint64_t *prior = nullptr;
+  if (refine) {
+    prior = ar.alloc<uint64_t>(batch * seg);
+    HIP_CHECK(hipMemcpyAsync(prior, key, sizeof(uint64_t) * batch * seg,
+                             hipMemcpyDeviceToDevice, s));
+  }
 
   // diagonals [m/4, ns) of each series (src/tile.cpp:12-38 keeps |i - j| <
   // m/4 out), rows and columns into one profile as the flags ask, after the
@@ -634,6 +674,10 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
     run_pass(ctx, ar, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor, 0,
              &sweep_ms, batch, seg, ns);
   }
+  if (refine)
+    HIP_CHECK(launch_refine(key, prior, ns, A.clean, A.mean, A.inorm, A.df, A.dg,
+                            ns, A.clean, A.mean, A.inorm, A.df, A.dg, m,
+                            RefineExclusion{m / 4, 0, true}, batch, seg, s));
 
   HIP_CHECK(launch_profile_out(key, seg, d_prof, mp, ns, batch, s));
   HIP_CHECK(hipEventRecord(ev_end.e, s));

@@ -63,7 +63,11 @@ typedef struct {
 
 /* Precisions: the SCAMPPrecisionType values (src/common.h:52-58).
  * DOUBLE (and ULTRA, which maps to it) = all-FP64 recurrence;
- * MIXED = FP32 recurrence re-anchored with FP64 every anchor_interval rows. */
+ * MIXED = FP32 recurrence re-anchored with FP64 every anchor_interval rows;
+ * its 1NN_INDEX entries that the join improved are then re-evaluated in FP64
+ * over the matched lag and the two lags on either side (no counterpart in
+ * the reference), so the stored correlation is the FP64 one and the index
+ * the FP64 form's wherever that lies within two lags. */
 enum {
   SCAMP_HIP_SINGLE = 1,
   SCAMP_HIP_MIXED = 2,
