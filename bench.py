@@ -179,7 +179,9 @@ def large_selfjoin(args):
             prof_max[r] = seg.max()
     prof_max = np.where(prof_max > 0.0, prof_max, np.float32(-2.0))  # cells keep values > 0
     row_max = np.max(np.where(np.isnan(msum), -2.0, msum), axis=1)
-    summary_rows_equal = bool(np.array_equal(row_max, prof_max))
+    # the summary keeps the sweep's FP32 values, the 1NN_INDEX profile its
+    # FP64-refined ones: they agree to the mixed form's rounding
+    summary_diff = float(np.max(np.abs(row_max.astype(np.float64) - prof_max)))
     # FP64 profile spot check with direct FP64 dot products
     rng = np.random.default_rng(5)
     worst = 0.0
@@ -199,7 +201,8 @@ def large_selfjoin(args):
                    "n": n, "window": m, "pairs": pairs, "anchor_interval": anchor},
         "joins": out,
         "mixed_max_abs_dcorr_vs_fp64": dcorr, "mixed_index_agreement_vs_fp64": agree,
-        "summary_row_max_equals_profile": summary_rows_equal,
+        "summary_row_max_vs_profile_max_abs_diff": summary_diff,
+        "summary_row_max_matches_profile": summary_diff <= 1e-6,
         "fp64_spot_check_max_abs_err_100_rows": worst,
     }
     print(json.dumps(line))
