@@ -52,11 +52,28 @@ def _compare(a, b, m, p64, p32, agree_min, mask=None):
     return agree
 
 
-def test_selfjoin_refined_to_the_fp64_choice():
-    a = random_walk(1 << 18, 41, nan_runs=8)
-    m = 256
+@pytest.mark.parametrize("n,m,agree_min", [
+    (1 << 18, 256, 0.9999),
+    # short series: many matches lie within two lags of either end, where the
+    # candidates are evaluated one at a time
+    (1500, 24, 0.995),
+    (700, 100, 0.995),
+])
+def test_selfjoin_refined_to_the_fp64_choice(n, m, agree_min):
+    a = random_walk(n, 41, nan_runs=8 if n > 10000 else 1)
     r = _both_forms(a, None, m)
-    _compare(a, None, m, r["double"][0], r["mixed"][0], 0.9999)
+    _compare(a, None, m, r["double"][0], r["mixed"][0], agree_min)
+
+
+def test_abjoin_short_b_edges():
+    """B barely longer than the window: most of A's matches sit within two
+    lags of B's ends."""
+    a = random_walk(5000, 45)
+    b = random_walk(70, 46)
+    m = 64
+    r = _both_forms(a, b, m, computing_rows=True, keep_rows_separate=True)
+    _compare(a, b, m, r["double"][0], r["mixed"][0], 0.995)
+    _compare(b, a, m, r["double"][1], r["mixed"][1], 0.995)
 
 
 def test_abjoin_refined_both_profiles():
