@@ -13,9 +13,11 @@
 // Every sample's gradient is gathered by one thread in a fixed order, with no
 // atomics, so the result is the same bit for bit from run to run:
 //   1. one thread per profile entry: r and three coefficients;
-//   2. the entries sorted by partner window (radix sort of (partner, entry));
+//   2. the entries sorted by partner window (radix sort of (partner, entry):
+//      per series in LDS for series up to 8192 windows, else device-wide);
 //   3. per sample t, the self half over the windows i that contain t, then
 //      the partner half over the entries whose partner window contains t.
+#include <rocprim/block/block_radix_sort.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "device_common.h"
@@ -49,7 +51,8 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_coef_kernel(
     const double *__restrict__ inorm, int64_t stat_stride,
     const int32_t *__restrict__ index, const double *__restrict__ grad_p,
     Coef *__restrict__ coef, uint32_t *__restrict__ key,
-    uint32_t *__restrict__ val) {
+    uint32_t *__restrict__ val, int shift) {
+  // shift > 0: per-series packed keys (partner << shift | entry, no values)
   const int64_t b = blockIdx.y;
   const int64_t i = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
   if (i >= ns) return;
@@ -60,8 +63,13 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_coef_kernel(
   const double nj = j >= 0 && j < ns ? inorm[b * stat_stride + j] : nan("");
   if (j < 0 || j >= ns || g == 0.0 || !(ni == ni) || !(nj == nj)) {
     coef[e] = Coef{0.0, 0.0, 0.0};
-    key[e] = (uint32_t)(batch * ns);  // after every real partner
-    val[e] = (uint32_t)e;
+    // after every real partner
+    if (shift > 0) {
+      key[e] = (uint32_t)((ns << shift) | i);
+    } else {
+      key[e] = (uint32_t)(batch * ns);
+      val[e] = (uint32_t)e;
+    }
     return;
   }
   const double *x = T + b * n + i;
@@ -79,8 +87,40 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_coef_kernel(
   for (; k < m; ++k) s0 = fma(x[k] - mi, y[k] - mj, s0);
   const double r = ((s0 + s1) + (s2 + s3)) * ni * nj;
   coef[e] = Coef{g * ni * nj, g * r * ni * ni, g * r * nj * nj};
-  key[e] = (uint32_t)(b * ns + j);
-  val[e] = (uint32_t)e;
+  if (shift > 0) {
+    key[e] = (uint32_t)((j << shift) | i);
+  } else {
+    key[e] = (uint32_t)(b * ns + j);
+    val[e] = (uint32_t)e;
+  }
+}
+
+// Pass 2 for series of at most kSeriesSortMax subsequences: one workgroup
+// sorts one series' packed keys in LDS.  The keys are unique, so the order is
+// (partner, entry), the same as the device-wide stable sort of (series,
+// partner) with entries as values gives.
+constexpr int kSeriesSortThreads = 1024;
+constexpr int kSeriesSortItems = 8;
+constexpr int64_t kSeriesSortMax = kSeriesSortThreads * kSeriesSortItems;
+
+__global__ __launch_bounds__(kSeriesSortThreads) void bwd_sort_series_kernel(
+    uint32_t *__restrict__ keys, int64_t ns, int end_bit) {
+  using Sort = rocprim::block_radix_sort<uint32_t, kSeriesSortThreads, kSeriesSortItems>;
+  __shared__ typename Sort::storage_type storage;
+  uint32_t *kb = keys + (int64_t)blockIdx.x * ns;
+  uint32_t k[kSeriesSortItems];
+  // blocked arrangement: thread t holds positions [t * items, (t + 1) * items)
+#pragma unroll
+  for (int q = 0; q < kSeriesSortItems; ++q) {
+    const int64_t r = (int64_t)threadIdx.x * kSeriesSortItems + q;
+    k[q] = r < ns ? kb[r] : 0xFFFFFFFFu;  // padding sorts last
+  }
+  Sort().sort(k, storage, 0u, static_cast<unsigned>(end_bit));
+#pragma unroll
+  for (int q = 0; q < kSeriesSortItems; ++q) {
+    const int64_t r = (int64_t)threadIdx.x * kSeriesSortItems + q;
+    if (r < ns) kb[r] = k[q];
+  }
 }
 
 // Pass 3a, one thread per sample t: the self half, a gather over the windows
@@ -167,7 +207,10 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
     const double *__restrict__ mean, int64_t stat_stride,
     const Coef *__restrict__ coef, const uint32_t *__restrict__ skey,
     const uint32_t *__restrict__ sval, int64_t nent,
-    double *__restrict__ grad_t) {
+    double *__restrict__ grad_t, int shift) {
+  // shift > 0: skey holds each series' packed keys (partner << shift | entry)
+  // sorted per series; otherwise all (series * ns + partner) keys sorted, with
+  // the entries in sval
   __shared__ double sP[kBwdChunk], sR[kBwdChunk], sMi[kBwdChunk], sMj[kBwdChunk];
   __shared__ int32_t sI[kBwdChunk], sJ[kBwdChunk];
   __shared__ int64_t range[2];
@@ -179,9 +222,15 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
   // partners reaching the block's samples: [t0-m+1, t0+255] within [0, ns)
   const int64_t jlo = t0 - m + 1 > 0 ? t0 - m + 1 : 0;
   const int64_t jhi = t0 + kBwdBlock - 1 < ns - 1 ? t0 + kBwdBlock - 1 : ns - 1;
+  if (shift > 0) skey += b * ns;
   if (threadIdx.x == 0) {
-    range[0] = lower_bound_u32(skey, nent, (uint32_t)(b * ns + jlo));
-    range[1] = lower_bound_u32(skey, nent, (uint32_t)(b * ns + jhi + 1));
+    if (shift > 0) {
+      range[0] = lower_bound_u32(skey, ns, (uint32_t)(jlo << shift));
+      range[1] = lower_bound_u32(skey, ns, (uint32_t)((jhi + 1) << shift));
+    } else {
+      range[0] = lower_bound_u32(skey, nent, (uint32_t)(b * ns + jlo));
+      range[1] = lower_bound_u32(skey, nent, (uint32_t)(b * ns + jhi + 1));
+    }
   }
   __syncthreads();
   const int64_t p0 = range[0], p1 = range[1];
@@ -190,9 +239,16 @@ __global__ __launch_bounds__(kBwdBlock) void bwd_partner_kernel(
     const int cn = (int)(p1 - c0 < kBwdChunk ? p1 - c0 : kBwdChunk);
     __syncthreads();
     for (int q = threadIdx.x; q < cn; q += kBwdBlock) {
-      const int64_t e = (int64_t)sval[c0 + q];
-      const int64_t i = e - b * ns;
-      const int64_t j = (int64_t)skey[c0 + q] - b * ns;
+      int64_t i, j;
+      if (shift > 0) {
+        const uint32_t k = skey[c0 + q];
+        j = k >> shift;
+        i = k & ((1u << shift) - 1u);
+      } else {
+        i = (int64_t)sval[c0 + q] - b * ns;
+        j = (int64_t)skey[c0 + q] - b * ns;
+      }
+      const int64_t e = b * ns + i;
       const Coef cf = coef[e];
       sP[q] = cf.P;
       sR[q] = cf.R;
@@ -297,19 +353,34 @@ hipError_t launch_pearson_backward(const double *T, int64_t n, int64_t m,
   if (err != hipSuccess) return err;
   BwdWork w = carve(work, nent, tmp);
   const unsigned nb = static_cast<unsigned>(batch);
+  // short series: packed keys sorted per series in LDS (one launch of
+  // `batch` workgroups instead of the device-wide sort's passes)
+  int shift = 0;
+  if (ns <= kSeriesSortMax) {
+    shift = 1;
+    while ((int64_t(1) << shift) <= ns) ++shift;  // partners 0..ns fit
+  }
   hipLaunchKernelGGL(bwd_coef_kernel,
                      dim3(static_cast<unsigned>((ns + kBwdBlock - 1) / kBwdBlock), nb),
                      dim3(kBwdBlock), 0, s, T, n, m, ns, batch, mean, inorm,
-                     stat_stride, index, grad_p, w.coef, w.key_in, w.val_in);
+                     stat_stride, index, grad_p, w.coef, w.key_in, w.val_in, shift);
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  err = sort_pairs(w.tmp, w.tmp_bytes, w.key_in, w.key_out, w.val_in, w.val_out,
-                   nent, bits, s);
-  if (err != hipSuccess) return err;
+  const uint32_t *skey = w.key_out;
+  if (shift > 0) {
+    hipLaunchKernelGGL(bwd_sort_series_kernel, dim3(nb), dim3(kSeriesSortThreads), 0, s,
+                       w.key_in, ns, 2 * shift);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    skey = w.key_in;
+  } else {
+    err = sort_pairs(w.tmp, w.tmp_bytes, w.key_in, w.key_out, w.val_in, w.val_out,
+                     nent, bits, s);
+    if (err != hipSuccess) return err;
+  }
   const dim3 grid(static_cast<unsigned>((n + kBwdBlock - 1) / kBwdBlock), nb);
   hipLaunchKernelGGL(bwd_self_kernel, grid, dim3(kBwdBlock), 0, s, T, n, m, ns,
                      mean, stat_stride, index, w.coef, grad_t);
   hipLaunchKernelGGL(bwd_partner_kernel, grid, dim3(kBwdBlock), 0, s, T, n, m, ns,
-                     mean, stat_stride, w.coef, w.key_out, w.val_out, nent, grad_t);
+                     mean, stat_stride, w.coef, skey, w.val_out, nent, grad_t, shift);
   return hipGetLastError();
 }
 
