@@ -83,7 +83,9 @@ __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
   double hi = 0, lo = 0;
   bool bad = false;
   for (int64_t t0 = 0; t0 < m; t0 += kStatChunk) {
-    const int64_t span = kStatBlock + kStatChunk;
+    const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
+    // the samples the block's windows read in this chunk
+    const int64_t span = kStatBlock + tend - 1;
     __syncthreads();
     for (int64_t j = threadIdx.x; j < span; j += kStatBlock) {
       int64_t g = w0 + t0 + j;
@@ -94,7 +96,6 @@ __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
       flag[j] = g < n && !isfinite(raw);
     }
     __syncthreads();
-    const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
     if (w < ns) {
       for (int64_t t = 0; t < tend; ++t) {
         const double v = tile[threadIdx.x + t];
@@ -109,14 +110,14 @@ __global__ __launch_bounds__(kStatBlock) void window_stats_kernel(
   const double mu = (hi + lo) / (double)m;
   double ssd = 0;
   for (int64_t t0 = 0; t0 < m; t0 += kStatChunk) {
-    const int64_t span = kStatBlock + kStatChunk;
+    const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
+    const int64_t span = kStatBlock + tend - 1;
     __syncthreads();
     for (int64_t j = threadIdx.x; j < span; j += kStatBlock) {
       int64_t g = w0 + t0 + j;
       tile[j] = g < n ? clean[g] : 0.0;
     }
     __syncthreads();
-    const int64_t tend = (m - t0) < kStatChunk ? (m - t0) : kStatChunk;
     if (w < ns) {
       for (int64_t t = 0; t < tend; ++t) {
         double d = tile[threadIdx.x + t] - mu;
