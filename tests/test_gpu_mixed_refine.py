@@ -106,3 +106,27 @@ def test_refinement_keeps_a_better_prior():
     rest = np.ones(ns, bool)
     rest[cols] = False
     _compare(a, None, m, ref, out, 0.9999, mask=rest)
+
+
+@pytest.mark.parametrize("grow,gcol", [(0, 0), (0, 131072), (131072, 0)])
+def test_aligned_abjoin_refinement_respects_exclusion(grow, gcol):
+    """An aligned AB-join of a series with itself excludes |global col -
+    global row| < m/4; the refined matches (lags j - 2 .. j + 2 of the sweep's)
+    must stay outside that zone, in both profiles, and agree with FP64."""
+    a = random_walk(6000, 62)
+    b = a.copy()
+    m = 64
+    e = m // 4
+    kw = dict(is_aligned=True, distributed_start_row=grow, distributed_start_col=gcol,
+              computing_rows=True, keep_rows_separate=True)
+    r = _both_forms(a, b, m, **kw)
+    delta = gcol - grow
+    for k, (src, dst) in enumerate(((a, b), (b, a))):
+        c32, i32 = C.split_mp_entry(r["mixed"][k])
+        ok = c32 > -1
+        rows = np.nonzero(ok)[0].astype(np.int64)
+        j = i32[ok].astype(np.int64)
+        # profile A (k = 0): A index = row, B index = match; profile B the reverse
+        d = (rows - j + delta) if k == 0 else (j - rows + delta)
+        assert np.all(np.abs(d) >= e)
+        _compare(src, dst, m, r["double"][k], r["mixed"][k], 0.999)
