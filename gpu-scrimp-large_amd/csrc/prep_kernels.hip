@@ -356,24 +356,28 @@ inline unsigned blocks_for(int64_t n, int bs) {
 
 
 // Warm start of a self-join's 1NN profile.  Thread i evaluates its window
-// against the kWarm windows just past the exclusion zone on each side,
-// i + d and i - d for d in [e, e + kWarm), and merges the best into its own
-// entries: forward partners (pairs where i is the row) into keyRow[i],
-// backward ones (i the column) into keyCol[i].  These are pairs of the join,
-// so the final profile is unchanged, but the sweep's thresholds start near
-// their final values instead of at "no match".  Covariances:
+// against the kWarm windows just past the exclusion zone ahead of it, j = i + d
+// for d in [e, e + kWarm), and each pair goes to both of its entries: the
+// row side of i (keyRow[i]) and the column side of j (keyCol[j]).  These are
+// pairs of the join, so the final profile is unchanged, but the sweep's
+// thresholds start near their final values instead of at "no match".
+// Covariances:
 //   sum_t x_t (T[j + t] - mu_i) - (mu_j - mu_i) sum_t x_t,  x_t = T[i + t] - mu_i
 // (both factors centred near the windows' level, so a large offset of the
 // series costs no digits); the correlation is formed in the sweep's order,
-// (cov * inorm_col) * inorm_row.  A block's windows, forward partners and
-// backward partners are staged through LDS, kWarmChunk samples of t at a
-// time; each thread slides register windows over the partners so that every
-// staged sample feeds kWarm FMAs.  One series per blockIdx.y (stride entries
-// apart); samples at or past n read as 0 and meet x = 0 or an invalid partner.
+// (cov * inorm_col) * inorm_row.  A block's windows and their partners are
+// staged through LDS, kWarmChunk samples of t at a time; each thread slides a
+// register window over the partners so that every staged sample feeds kWarm
+// FMAs.  The column sides of the block's pairs meet in an LDS table (one key
+// per partner window) before one atomicMax per partner; the row sides go out
+// with atomicMax too, as both may land in one profile.  One series per
+// blockIdx.y (stride entries apart); samples at or past n read as 0 and meet
+// x = 0 or an invalid partner.
 constexpr int kWarm = 8;
 constexpr int kWarmBlock = 256;
 constexpr int kWarmChunk = 256;
 constexpr int kWarmStage = kWarmBlock + kWarmChunk + 2 * kWarm;
+constexpr int kWarmCols = kWarmBlock + kWarm - 1;  // partners i0 + e + [0, kWarmCols)
 
 __global__ __launch_bounds__(kWarmBlock) void warm_kernel(
     const double *__restrict__ T, const double *__restrict__ mu,
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(kWarmBlock) void warm_kernel(
     int64_t e, int64_t stride, uint64_t *keyCol, uint64_t *keyRow) {
   __shared__ double sx[kWarmStage];  // T[i0 + t0 + q]
   __shared__ double sf[kWarmStage];  // T[i0 + e + t0 + q]
-  __shared__ double sb[kWarmStage];  // T[i0 - e - (kWarm - 1) + t0 + q]
+  __shared__ unsigned long long scol[kWarmCols];
   const int j = threadIdx.x;
   const int64_t i0 = blockIdx.x * (int64_t)kWarmBlock;
   const int64_t i = i0 + j;
@@ -389,79 +393,66 @@ __global__ __launch_bounds__(kWarmBlock) void warm_kernel(
   T += o;
   mu += o;
   inorm += o;
+  for (int q = j; q < kWarmCols; q += kWarmBlock) scol[q] = 0;
   const double ini = i < ns ? inorm[i] : nan("");
   const double mi = i < ns ? mu[i] : 0.0;
-  double f[kWarm], b[kWarm];  // partners i + e + w and i - e - w
+  double f[kWarm];  // partners i + e + w
 #pragma unroll
-  for (int w = 0; w < kWarm; ++w) f[w] = b[w] = 0.0;
+  for (int w = 0; w < kWarm; ++w) f[w] = 0.0;
   double sxs = 0.0;
   for (int64_t t0 = 0; t0 < m; t0 += kWarmChunk) {
     const int cl = (int)(m - t0 < kWarmChunk ? m - t0 : kWarmChunk);
     __syncthreads();
     for (int q = j; q < kWarmStage; q += kWarmBlock) {
-      const int64_t ix = i0 + t0 + q, jf = ix + e, jb = ix - e - (kWarm - 1);
+      const int64_t ix = i0 + t0 + q, jf = ix + e;
       sx[q] = ix < n ? T[ix] : 0.0;
       sf[q] = jf < n ? T[jf] : 0.0;
-      sb[q] = jb >= 0 && jb < n ? T[jb] : 0.0;
     }
     __syncthreads();
-    // fw[q] = T[i + e + tt + q] - mi, bw[q] = T[i - e - (kWarm-1) + tt + q] - mi
-    double fw[2 * kWarm - 1], bw[2 * kWarm - 1];
+    // fw[q] = T[i + e + tt + q] - mi
+    double fw[2 * kWarm - 1];
 #pragma unroll
-    for (int q = 0; q < kWarm - 1; ++q) {
-      fw[kWarm + q] = sf[j + q] - mi;
-      bw[kWarm + q] = sb[j + q] - mi;
-    }
+    for (int q = 0; q < kWarm - 1; ++q) fw[kWarm + q] = sf[j + q] - mi;
     for (int tt = 0; tt < cl; tt += kWarm) {
 #pragma unroll
-      for (int q = 0; q < kWarm - 1; ++q) {
-        fw[q] = fw[kWarm + q];
-        bw[q] = bw[kWarm + q];
-      }
+      for (int q = 0; q < kWarm - 1; ++q) fw[q] = fw[kWarm + q];
 #pragma unroll
-      for (int q = 0; q < kWarm; ++q) {
-        fw[kWarm - 1 + q] = sf[j + tt + kWarm - 1 + q] - mi;
-        bw[kWarm - 1 + q] = sb[j + tt + kWarm - 1 + q] - mi;
-      }
+      for (int q = 0; q < kWarm; ++q) fw[kWarm - 1 + q] = sf[j + tt + kWarm - 1 + q] - mi;
 #pragma unroll
       for (int u = 0; u < kWarm; ++u) {
         const double x = tt + u < cl ? sx[j + tt + u] - mi : 0.0;
         sxs += x;
 #pragma unroll
-        for (int w = 0; w < kWarm; ++w) {
-          f[w] = fma(x, fw[u + w], f[w]);              // T[i+e+w+t] - mi
-          b[w] = fma(x, bw[u + kWarm - 1 - w], b[w]);  // T[i-e-w+t] - mi
-        }
+        for (int w = 0; w < kWarm; ++w) f[w] = fma(x, fw[u + w], f[w]);  // T[i+e+w+t] - mi
       }
     }
   }
-  if (i >= ns || !(ini == ini)) return;  // no window, or one without matches
-  uint64_t best_row = 0, best_col = 0;
+  uint64_t best_row = 0;
+  if (i < ns && ini == ini) {  // a window with matches
 #pragma unroll
-  for (int w = 0; w < kWarm; ++w) {
-    const int64_t jf = i + e + w, jb = i - e - w;
-    if (keyRow && jf < ns) {
+    for (int w = 0; w < kWarm; ++w) {
+      const int64_t jf = i + e + w;
+      if (jf >= ns) continue;
+      // column jf, row i
       const double c = ((f[w] - (mu[jf] - mi) * sxs) * inorm[jf]) * ini;
-      if (c == c) {
-        const uint64_t k = make_key((float)c, (uint32_t)jf);
-        best_row = k > best_row ? k : best_row;
-      }
-    }
-    if (keyCol && jb >= 0) {
-      const double c = ((b[w] - (mu[jb] - mi) * sxs) * ini) * inorm[jb];
-      if (c == c) {
-        const uint64_t k = make_key((float)c, (uint32_t)jb);
-        best_col = k > best_col ? k : best_col;
-      }
+      if (!(c == c)) continue;  // NaN or flat partner
+      const uint64_t kr = make_key((float)c, (uint32_t)jf);
+      best_row = kr > best_row ? kr : best_row;
+      if (keyCol)
+        atomicMax(&scol[j + w], static_cast<unsigned long long>(
+                                    make_key((float)c, (uint32_t)i)));
     }
   }
-  if (keyRow == keyCol) {
-    const uint64_t best = best_row > best_col ? best_row : best_col;
-    uint64_t *p = keyRow + o + i;
-    if (best > *p) *p = best;
-  } else {
-    if (keyRow && best_row > keyRow[o + i]) keyRow[o + i] = best_row;
-    if (keyCol && best_col > keyCol[o + i]) keyCol[o + i] = best_col;
+  if (keyRow && best_row != 0)
+    atomicMax(reinterpret_cast<unsigned long long *>(keyRow + o + i),
+              static_cast<unsigned long long>(best_row));
+  if (!keyCol) return;
+  __syncthreads();
+  for (int q = j; q < kWarmCols; q += kWarmBlock) {
+    const int64_t jf = i0 + e + q;
+    const unsigned long long v = scol[q];
+    if (v != 0 && jf < ns)
+      atomicMax(reinterpret_cast<unsigned long long *>(keyCol + o + jf), v);
   }
 }
 
