@@ -72,19 +72,26 @@ hipError_t launch_warm_profile(const double *clean, const double *mean,
 // Pairs (u, v) with u an index of the profile's series U and v of V are in
 // the join unless zone > 0 and |a - b + delta| < zone, where a and b are
 // the A and B indices of the pair (u_is_a: u is the A index).  A self-join
-// passes zone = m/4, delta = 0, u_is_a = true.
+// passes zone = m/4, delta = 0, u_is_a = true, and `side` says which of
+// its pairs the profile holds: 0 both, -1 only partners v <= u - zone (the
+// profile of columns: pairs on diagonals j - i >= zone reach column j from
+// rows i below it), +1 only partners v >= u + zone (the profile of rows).
 struct RefineExclusion {
   int64_t zone;
   int64_t delta;
   bool u_is_a;
+  int32_t side;
 };
+// `drift` (one counter) counts the refined entries whose FP32 sweep value
+// differs from the FP64 correlation of the same pair by more than
+// `drift_tol`: the FP32 recurrence went beyond its error bound there.
 hipError_t launch_refine(uint64_t *key, const uint64_t *prior, int64_t nsU,
                          const double *TU, const double *muU, const double *inU,
                          const double *dfU, const double *dgU, int64_t nsV,
                          const double *TV, const double *muV, const double *inV,
                          const double *dfV, const double *dgV, int64_t m,
                          RefineExclusion ex, int64_t batch, int64_t stride,
-                         hipStream_t s);
+                         uint64_t *drift, double drift_tol, hipStream_t s);
 
 // ---- backward of the differentiable join (backward_kernel.hip) ------------
 // grad_t[b, :] = sum over profile entries i of grad_p[b, i] * d corr(i, I_i)

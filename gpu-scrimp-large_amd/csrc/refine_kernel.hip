@@ -19,6 +19,12 @@
 // thread's chunk computes them directly, m FMAs per candidate.
 // This pass has no counterpart in the reference (its GPU mixed form reports
 // the FP32 values as computed).
+//
+// It also audits the FP32 sweep: an entry whose sweep value differs from the
+// FP64 correlation of the same pair by more than `drift_tol` is counted in
+// `drift`.  Such drift comes from inputs whose amplitude spans many orders of
+// magnitude (a loud stretch leaves an FP32 rounding error behind that later
+// quiet windows magnify); the caller then recomputes the join in FP64.
 #include "device_common.h"
 #include "kernels.h"
 
@@ -37,7 +43,8 @@ __global__ __launch_bounds__(kRefineBlock) void refine_kernel(
     const double *__restrict__ dgU, int64_t nsV, const double *__restrict__ TV,
     const double *__restrict__ muV, const double *__restrict__ inV,
     const double *__restrict__ dfV, const double *__restrict__ dgV, int64_t m,
-    RefineExclusion ex, int64_t stride) {
+    RefineExclusion ex, int64_t stride, uint64_t *__restrict__ drift,
+    double drift_tol) {
   // batched self-joins: series blockIdx.y, every array `stride` entries on
   const int64_t o = (int64_t)blockIdx.y * stride;
   key += o;
@@ -115,6 +122,13 @@ __global__ __launch_bounds__(kRefineBlock) void refine_kernel(
     }
     uint64_t best = kp;
     const double iu = inU[i];
+    // the sweep's own pair j, recomputed: how far did the FP32 value drift?
+    if (have[kLags]) {
+      const double c64 = cov[kLags] * inV[j] * iu;
+      const double c32 = static_cast<double>(key_corr(k));
+      if (fabs(c64 - c32) > drift_tol)
+        atomicAdd(reinterpret_cast<unsigned long long *>(drift), 1ull);
+    }
     for (int q = 0; q < kCand; ++q) {
       const int64_t v = j - kLags + q;
       if (!have[q]) continue;
@@ -123,6 +137,8 @@ __global__ __launch_bounds__(kRefineBlock) void refine_kernel(
       // the join's exclusion zone (u, v as A / B indices)
       const int64_t d = ex.u_is_a ? i - v + ex.delta : v - i + ex.delta;
       if (ex.zone > 0 && (d < ex.zone && d > -ex.zone)) continue;
+      // one-sided self-join profiles hold the partners on one side only
+      if ((ex.side < 0 && v > i - ex.zone) || (ex.side > 0 && v < i + ex.zone)) continue;
       const double corr = cov[q] * iv * iu;
       const uint64_t kc = make_key(static_cast<float>(corr), static_cast<uint32_t>(v));
       best = kc > best ? kc : best;
@@ -140,13 +156,14 @@ hipError_t launch_refine(uint64_t *key, const uint64_t *prior, int64_t nsU,
                          const double *TV, const double *muV, const double *inV,
                          const double *dfV, const double *dgV, int64_t m,
                          RefineExclusion ex, int64_t batch, int64_t stride,
-                         hipStream_t s) {
+                         uint64_t *drift, double drift_tol, hipStream_t s) {
   if (nsU <= 0 || batch <= 0) return hipSuccess;
   const int64_t threads = (nsU + kRefineChunk - 1) / kRefineChunk;
   const unsigned blocks = static_cast<unsigned>((threads + kRefineBlock - 1) / kRefineBlock);
   hipLaunchKernelGGL(refine_kernel, dim3(blocks, static_cast<unsigned>(batch)),
                      dim3(kRefineBlock), 0, s, key, prior, nsU, TU, muU, inU, dfU,
-                     dgU, nsV, TV, muV, inV, dfV, dgV, m, ex, stride);
+                     dgU, nsV, TV, muV, inV, dfV, dgV, m, ex, stride, drift,
+                     drift_tol);
   return hipGetLastError();
 }
 

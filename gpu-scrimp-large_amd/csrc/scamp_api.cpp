@@ -11,7 +11,6 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
-#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -108,25 +107,29 @@ std::string getSCAMPErrorString(SCAMPError_t err) {
 
 namespace {
 
-// One context per device, created on first use and kept for the process.
-scamp_hip_ctx *context_for(int device) {
-  static std::mutex mu;
-  static std::vector<scamp_hip_ctx *> ctxs;
-  std::lock_guard<std::mutex> lock(mu);
-  if (device < 0) throw SCAMPException("Error: invalid device id");
-  if ((size_t)device >= ctxs.size()) ctxs.resize(device + 1, nullptr);
-  if (!ctxs[device]) {
-    scamp_hip_ctx *c = nullptr;
-    int rc = scamp_hip_ctx_create(device, &c);
-    if (rc != SCAMP_HIP_OK) {
+// The device state of one do_SCAMP call: a context (stream + device blocks)
+// created on entry and destroyed on every exit path, so the call leaves no
+// device memory behind and concurrent calls from different host threads run
+// on their own streams, as the reference's per-call workers and tiles do
+// (src/SCAMP.cpp:240-256, src/common.cpp:109-124).
+class CallContext {
+ public:
+  explicit CallContext(int device) {
+    if (device < 0) throw SCAMPException("Error: invalid device id");
+    if (scamp_hip_ctx_create(device, &ctx_) != SCAMP_HIP_OK) {
       char buf[512];
       scamp_hip_last_error(buf, sizeof(buf));
       throw SCAMPException(std::string("Error: no HIP device usable: ") + buf);
     }
-    ctxs[device] = c;
   }
-  return ctxs[device];
-}
+  ~CallContext() { scamp_hip_ctx_destroy(ctx_); }
+  CallContext(const CallContext &) = delete;
+  CallContext &operator=(const CallContext &) = delete;
+  scamp_hip_ctx *get() const { return ctx_; }
+
+ private:
+  scamp_hip_ctx *ctx_ = nullptr;
+};
 
 void *profile_buffer(Profile &p, SCAMPProfileType t) {
   if (p.data.empty()) return nullptr;
@@ -206,7 +209,6 @@ void do_SCAMP(SCAMPArgs *args, const std::vector<int> &devices,
                          " executing tile");
   }
   const int device = devices.empty() ? 0 : devices[0];
-  scamp_hip_ctx *ctx = context_for(device);
 
   const size_t na = args->timeseries_a.size();
   const size_t nb = args->has_b ? args->timeseries_b.size() : na;
@@ -243,6 +245,8 @@ void do_SCAMP(SCAMPArgs *args, const std::vector<int> &devices,
   p.anchor_interval = args->anchor_interval;
   p.max_matches_per_column = args->max_matches_per_column;
 
+  CallContext call(device);
+  scamp_hip_ctx *ctx = call.get();
   int rc;
   if (t == PROFILE_TYPE_APPROX_ALL_NEIGHBORS) {
     const int64_t k = args->max_matches_per_column;

@@ -36,6 +36,10 @@ struct scamp_hip_ctx {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   float total_ms = 0, prep_ms = 0, sweep_ms = 0;
+  // the precision the last join ran in, and the entries whose FP32 sweep
+  // value failed the mixed form's audit (then the join ran again in FP64)
+  int last_precision = 0;
+  uint64_t last_drift = 0;
   std::mutex mu;
   // Device blocks kept from earlier calls (bytes, pointer), reused by later
   // ones: a short join would otherwise spend a noticeable part of its time
@@ -50,6 +54,31 @@ struct scamp_hip_ctx {
 namespace {
 
 thread_local std::string g_last_error;
+
+// The mixed form's audit: an entry whose FP32 sweep correlation is further
+// than this from the FP64 correlation of the same pair (checked by the
+// refinement pass) means the FP32 recurrence left its error bound, and the
+// join is recomputed in FP64.  While every pair's FP32 value is within
+// kDriftTol of its FP64 one, the match the sweep keeps is within 2 kDriftTol
+// = 1e-5 of the true best, and the refinement stores its exact value.  The
+// compensated sweep stays below 1e-6 on well-conditioned series at the
+// default anchor interval.
+constexpr double kDriftTol = 5e-6;
+
+// Rows between the mixed form's FP64 re-anchors when the caller leaves it
+// to the engine: 2 m, within [512, 16384].  An anchor costs m / R FP64 FMAs
+// per pair (at most 0.5 here); the FP32 drift grows with the rows since the
+// last one, and series with loud stretches (the planted high-amplitude
+// motifs of an AB-join, for one) need short intervals to stay within the
+// audit's bound.  Callers that tune R for their data pass it explicitly.
+int64_t anchor_rows(const scamp_hip_params *p) {
+  int64_t r = p->anchor_interval > 0
+                  ? p->anchor_interval
+                  : std::min<int64_t>(16384, std::max<int64_t>(512, 2 * p->window));
+  return std::max<int64_t>(16, (r + 15) / 16 * 16);
+}
+// join_device / selfjoin_batch: the audit failed; run again in FP64
+constexpr int kRerunFp64 = -1;
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -400,8 +429,7 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   if (pt == SCAMP_HIP_MATRIX_SUMMARY &&
       (p->matrix_height <= 0 || p->matrix_width <= 0))
     return fail(SCAMP_HIP_INVALID_ARGS, "matrix height and width must be > 0");
-  int64_t anchor = p->anchor_interval > 0 ? p->anchor_interval : 8192;
-  anchor = std::max<int64_t>(16, (anchor + 15) / 16 * 16);
+  const int64_t anchor = anchor_rows(p);
 
   hipStream_t s = ctx->stream;
   DevArena ar(ctx);
@@ -471,8 +499,10 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   // The mixed form's 1NN_INDEX profiles are refined in FP64 after the sweep
   // (refine_kernel.hip); entries still equal to the prior are left alone.
   const bool refine = fp32 && pt == SCAMP_HIP_1NN_INDEX;
-  uint64_t *priorA = nullptr, *priorB = nullptr;
+  uint64_t *priorA = nullptr, *priorB = nullptr, *drift = nullptr;
   if (refine) {
+    drift = ar.alloc<uint64_t>(1);
+    HIP_CHECK(launch_fill_u64(drift, 1, 0, s));
     priorA = ar.alloc<uint64_t>(nsA);
     HIP_CHECK(hipMemcpyAsync(priorA, keyA, sizeof(uint64_t) * nsA,
                              hipMemcpyDeviceToDevice, s));
@@ -561,13 +591,41 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
         delta = p->distributed_start_col - p->distributed_start_row;
       zone = (p->is_aligned != 0 && e > 0) ? e : 0;
     }
+    // a self-join's profile A holds column pairs (partners below), row pairs
+    // (partners above), or both, as the passes above wrote it
+    int32_t side_a = 0, side_b = 0;
+    if (self) {
+      side_a = cols_out && rows_out && !sep_rows ? 0 : (cols_out ? -1 : 1);
+      side_b = 1;
+    }
     HIP_CHECK(launch_refine(keyA, priorA, nsA, A.clean, A.mean, A.inorm, A.df,
                             A.dg, nsB, B.clean, B.mean, B.inorm, B.df, B.dg, m,
-                            RefineExclusion{zone, delta, true}, 1, 0, s));
+                            RefineExclusion{zone, delta, true, side_a}, 1, 0, drift,
+                            kDriftTol, s));
     if (sep_rows)
       HIP_CHECK(launch_refine(keyB, priorB, nsB, B.clean, B.mean, B.inorm, B.df,
                               B.dg, nsA, A.clean, A.mean, A.inorm, A.df, A.dg, m,
-                              RefineExclusion{zone, delta, false}, 1, 0, s));
+                              RefineExclusion{zone, delta, false, side_b}, 1, 0, drift,
+                              kDriftTol, s));
+    uint64_t h_drift = 0;
+    HIP_CHECK(hipMemcpyAsync(&h_drift, drift, sizeof(h_drift), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    ctx->last_drift = h_drift;
+    if (h_drift > 0) {
+      // the caller's profiles are untouched so far (the result lives in the
+      // working keys): the join runs again in FP64
+      float spent = 0;
+      HIP_CHECK(hipEventRecord(ev_end.e, s));
+      HIP_CHECK(hipEventSynchronize(ev_end.e));
+      HIP_CHECK(hipEventElapsedTime(&spent, ev_start.e, ev_end.e));
+      ctx->total_ms = spent;
+      ctx->sweep_ms = sweep_ms;
+      if (!p->silent_mode)
+        std::printf("scamp_hip: the mixed form's FP32 sweep drifted beyond %.0e on "
+                    "%llu entries; recomputing in FP64\n",
+                    kDriftTol, (unsigned long long)h_drift);
+      return kRerunFp64;
+    }
   }
 
   // Back to SCAMP's layouts.
@@ -599,6 +657,8 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
   ctx->total_ms = total;
   ctx->sweep_ms = sweep_ms;
   ctx->prep_ms = total - sweep_ms;
+  ctx->last_precision = prec;
+  if (!fp32) ctx->last_drift = 0;
   if (!p->silent_mode)
     std::printf("scamp_hip: join of %lld x %lld subsequences in %.3f ms "
                 "(sweep %.3f ms)\n",
@@ -617,8 +677,7 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   const int pt = p->profile_type;
   const int prec = p->precision;
   const bool fp32 = prec == SCAMP_HIP_MIXED || prec == SCAMP_HIP_SINGLE;
-  int64_t anchor = p->anchor_interval > 0 ? p->anchor_interval : 8192;
-  anchor = std::max<int64_t>(16, (anchor + 15) / 16 * 16);
+  const int64_t anchor = anchor_rows(p);
   const int64_t ns = n - m + 1;
   const int64_t seg = (n + kColumnPad + 63) / 64 * 64;
   hipStream_t s = ctx->stream;
@@ -653,8 +712,10 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   HIP_CHECK(launch_profile_in(d_prof, mp, key, ns, batch, seg, s));
   // mixed 1NN_INDEX: FP64 refinement after the sweep, as in a single join
   const bool refine = fp32 && mp;
-  uint64_t *prior = nullptr;
+  uint64_t *prior = nullptr, *drift = nullptr;
   if (refine) {
+    drift = ar.alloc<uint64_t>(1);
+    HIP_CHECK(launch_fill_u64(drift, 1, 0, s));
     prior = ar.alloc<uint64_t>(batch * seg);
     HIP_CHECK(hipMemcpyAsync(prior, key, sizeof(uint64_t) * batch * seg,
                              hipMemcpyDeviceToDevice, s));
@@ -674,10 +735,30 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
     run_pass(ctx, ar, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor, 0,
              &sweep_ms, batch, seg, ns);
   }
-  if (refine)
+  if (refine) {
     HIP_CHECK(launch_refine(key, prior, ns, A.clean, A.mean, A.inorm, A.df, A.dg,
                             ns, A.clean, A.mean, A.inorm, A.df, A.dg, m,
-                            RefineExclusion{m / 4, 0, true}, batch, seg, s));
+                            RefineExclusion{m / 4, 0, true,
+                                            cols && rows ? 0 : (cols ? -1 : 1)},
+                            batch, seg, drift, kDriftTol, s));
+    uint64_t h_drift = 0;
+    HIP_CHECK(hipMemcpyAsync(&h_drift, drift, sizeof(h_drift), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    ctx->last_drift = h_drift;
+    if (h_drift > 0) {
+      float spent = 0;
+      HIP_CHECK(hipEventRecord(ev_end.e, s));
+      HIP_CHECK(hipEventSynchronize(ev_end.e));
+      HIP_CHECK(hipEventElapsedTime(&spent, ev_start.e, ev_end.e));
+      ctx->total_ms = spent;
+      ctx->sweep_ms = sweep_ms;
+      if (!p->silent_mode)
+        std::printf("scamp_hip: the mixed form's FP32 sweep drifted beyond %.0e on "
+                    "%llu entries; recomputing in FP64\n",
+                    kDriftTol, (unsigned long long)h_drift);
+      return kRerunFp64;
+    }
+  }
 
   HIP_CHECK(launch_profile_out(key, seg, d_prof, mp, ns, batch, s));
   HIP_CHECK(hipEventRecord(ev_end.e, s));
@@ -687,11 +768,31 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   ctx->total_ms = total;
   ctx->sweep_ms = sweep_ms;
   ctx->prep_ms = total - sweep_ms;
+  ctx->last_precision = prec;
+  if (!fp32) ctx->last_drift = 0;
   if (!p->silent_mode)
     std::printf("scamp_hip: %lld self-joins of %lld subsequences in %.3f ms "
                 "(sweep %.3f ms)\n",
                 (long long)batch, (long long)ns, total, sweep_ms);
   return SCAMP_HIP_OK;
+}
+
+// Runs `run(params)`; when the mixed form's audit asks for it, runs again
+// with the precision set to DOUBLE, and reports the time of both runs.
+template <typename Run>
+int with_fp64_rerun(scamp_hip_ctx *ctx, const scamp_hip_params *p, Run run) {
+  int rc = run(p);
+  if (rc != kRerunFp64) return rc;
+  const float t0 = ctx->total_ms, s0 = ctx->sweep_ms;
+  const uint64_t drift = ctx->last_drift;
+  scamp_hip_params p64 = *p;
+  p64.precision = SCAMP_HIP_DOUBLE;
+  rc = run(&p64);
+  ctx->total_ms += t0;
+  ctx->sweep_ms += s0;
+  ctx->prep_ms = ctx->total_ms - ctx->sweep_ms;
+  ctx->last_drift = drift;
+  return rc;
 }
 
 size_t profile_elem_bytes(int pt) {
@@ -771,8 +872,10 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, p, d_profile_a,
-                       d_profile_b);
+    return with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
+      return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, q, d_profile_a,
+                         d_profile_b);
+    });
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
     int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
@@ -808,7 +911,9 @@ int scamp_hip_selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T,
   try {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    return selfjoin_batch(ctx, d_T, (int64_t)n, batch, p, d_profiles);
+    return with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
+      return selfjoin_batch(ctx, d_T, (int64_t)n, batch, q, d_profiles);
+    });
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
     int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
@@ -855,7 +960,9 @@ int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
       d_pb = ar.alloc<char>((int64_t)(eb * len_b));
       HIP_CHECK(hipMemcpyAsync(d_pb, profile_b, eb * len_b, hipMemcpyHostToDevice, s));
     }
-    int rc = join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, p, d_pa, d_pb);
+    int rc = with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
+      return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, q, d_pa, d_pb);
+    });
     if (rc != SCAMP_HIP_OK) return rc;
     HIP_CHECK(hipMemcpyAsync(profile_a, d_pa, eb * len_a, hipMemcpyDeviceToHost, s));
     if (sep)
@@ -978,6 +1085,15 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
     return fail(SCAMP_HIP_DEVICE_ERROR,
                 std::string(f.what) + ": " + hipGetErrorString(f.err));
   }
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_last_precision(scamp_hip_ctx *ctx, int *precision,
+                             uint64_t *drift_entries) {
+  if (!ctx) return fail(SCAMP_HIP_INVALID_ARGS, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (precision) *precision = ctx->last_precision;
+  if (drift_entries) *drift_entries = ctx->last_drift;
   return SCAMP_HIP_OK;
 }
 

@@ -28,7 +28,8 @@ STATUS = {0: "OK", 1: "FUNCTIONALITY_UNIMPLEMENTED", 2: "ILLEGAL_TYPE",
 EXPORTS = ("scamp_hip_ctx_create", "scamp_hip_ctx_destroy", "scamp_hip_device_count",
            "scamp_hip_last_error", "scamp_hip_join", "scamp_hip_join_device",
            "scamp_hip_stats", "scamp_hip_qt_seed", "scamp_hip_last_timing",
-           "scamp_hip_pearson_backward", "scamp_hip_selfjoin_batch")
+           "scamp_hip_pearson_backward", "scamp_hip_selfjoin_batch",
+           "scamp_hip_last_precision")
 
 
 class Params(ctypes.Structure):
@@ -84,6 +85,8 @@ def load():
     lib.scamp_hip_last_timing.argtypes = [vp, dp, dp, dp]
     lib.scamp_hip_pearson_backward.argtypes = [vp, vp, sz, i64, i64, vp, vp, vp]
     lib.scamp_hip_selfjoin_batch.argtypes = [vp, vp, sz, i64, ctypes.POINTER(Params), vp]
+    lib.scamp_hip_last_precision.argtypes = [vp, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_uint64)]
     for name in EXPORTS:
         if hasattr(lib, name):
             getattr(lib, name).restype = ctypes.c_int
@@ -137,6 +140,15 @@ class Context:
         check(self._lib.scamp_hip_last_timing(self._h, ctypes.byref(t), ctypes.byref(p),
                                               ctypes.byref(s)))
         return {"total_ms": t.value, "prep_ms": p.value, "sweep_ms": s.value}
+
+    def last_precision(self):
+        """(precision name the last join ran in, entries that failed the
+        mixed form's FP64 audit)."""
+        prec, drift = ctypes.c_int(), ctypes.c_uint64()
+        check(self._lib.scamp_hip_last_precision(self._h, ctypes.byref(prec),
+                                                 ctypes.byref(drift)))
+        names = {v: k for k, v in PRECISIONS.items()}
+        return names.get(prec.value, prec.value), drift.value
 
 
 _ctx_cache = {}

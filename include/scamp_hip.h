@@ -62,12 +62,16 @@ typedef struct {
 } scamp_hip_match;
 
 /* Precisions: the SCAMPPrecisionType values (src/common.h:52-58).
- * DOUBLE (and ULTRA, which maps to it) = all-FP64 recurrence;
- * MIXED = FP32 recurrence re-anchored with FP64 every anchor_interval rows;
- * its 1NN_INDEX entries that the join improved are then re-evaluated in FP64
- * over the matched lag and the two lags on either side (no counterpart in
- * the reference), so the stored correlation is the FP64 one and the index
- * the FP64 form's wherever that lies within two lags. */
+ * DOUBLE (and ULTRA, which maps to it: the statistics are direct window sums
+ * either way) = all-FP64 recurrence;
+ * MIXED (and SINGLE, which names the same form) = compensated FP32
+ * recurrence re-anchored with FP64 every anchor_interval rows; its 1NN_INDEX
+ * entries that the join improved are then re-evaluated in FP64 over the
+ * matched lag and the two lags on either side (no counterpart in the
+ * reference), so the stored correlation is the FP64 one and the index the
+ * FP64 form's wherever that lies within two lags.  The same pass audits the
+ * FP32 sweep: when a matched pair's FP32 correlation is off by more than
+ * 5e-6, the join is recomputed in FP64 (scamp_hip_last_precision). */
 enum {
   SCAMP_HIP_SINGLE = 1,
   SCAMP_HIP_MIXED = 2,
@@ -96,7 +100,8 @@ typedef struct {
   int64_t matrix_width;
   int64_t max_tile_size;       /* validated as SCAMP does; the sweep is untiled */
   int64_t anchor_interval;     /* MIXED only: rows between FP64 re-anchors,
-                                  0 = library default (8192) */
+                                  0 = library default (2 * window, within
+                                  [512, 16384]) */
   int64_t max_matches_per_column; /* ALL_NEIGHBORS: k, 1..256 */
 } scamp_hip_params;
 
@@ -183,6 +188,15 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
  * total, statistics+seeds, and the diagonal sweep, in milliseconds. */
 int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
                           double *prep_ms, double *sweep_ms);
+
+/* The precision the last join on this context ran in (a scamp_hip_precision
+ * value) and the number of 1NN_INDEX entries whose FP32 sweep correlation
+ * failed the mixed form's FP64 audit (0 unless MIXED/SINGLE).  A nonzero
+ * count means the join was recomputed in FP64 and precision reads DOUBLE.
+ * No counterpart in the reference, whose mixed mode has no audit
+ * (docs/source/cli.rst:127 lists inputs on which it fails). */
+int scamp_hip_last_precision(scamp_hip_ctx *ctx, int *precision,
+                             uint64_t *drift_entries);
 
 #ifdef __cplusplus
 }
