@@ -22,6 +22,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PTYPES = {"1NN_INDEX": 1, "SUM_THRESH": 2, "1NN": 6}
+# SCAMPPrecisionType (src/common.h:53-58) of the modes SCAMP's CPU path runs
+PRECISIONS = {"double": 3, "ultra": 4}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int64)
@@ -84,7 +86,7 @@ def _load_ref():
         lib.ref_join.argtypes = [
             _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
             ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-            ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+            ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_int, ctypes.c_int,
             _dp, _ip, _dp, _ip]
         lib.ref_join.restype = ctypes.c_int
         _ref = lib
@@ -125,13 +127,15 @@ def _alloc(ptype, n):
 def join(a, b, m, profile_type="1NN_INDEX", max_tile_size=1 << 17,
          computing_rows=None, computing_cols=True, keep_rows_separate=False,
          is_aligned=False, global_start_row=-1, global_start_col=-1,
-         threshold=0.0, num_workers=1, fp64=False, impl="port"):
+         threshold=0.0, num_workers=1, fp64=False, impl="port", precision="double"):
     """Run a join through the restatement (impl="port") or SCAMP's own
     compiled CPU path (impl="reference").
 
     Returns (corr_a, idx_a) -- plus (corr_b, idx_b) when keep_rows_separate.
     ``idx`` is None for 1NN and SUM_THRESH.  computing_rows defaults to the
-    pyscamp choice: True for self-joins, False for AB-joins.
+    pyscamp choice: True for self-joins, False for AB-joins.  ``precision``
+    ("double" or "ultra") selects SCAMP's CPU precision mode and applies to
+    impl="reference" only; the restatement follows the DOUBLE path.
     """
     a = np.ascontiguousarray(a, dtype=np.float64)
     has_b = b is not None
@@ -146,6 +150,8 @@ def join(a, b, m, profile_type="1NN_INDEX", max_tile_size=1 << 17,
     ca, ia = _alloc(profile_type, na - m + 1)
     cb, ib = _alloc(profile_type, nb - m + 1) if keep_rows_separate else (None, None)
     pt = PTYPES[profile_type]
+    if precision != "double" and impl != "reference":
+        raise ValueError("the restatement follows SCAMP's DOUBLE path only")
     if impl == "reference":
         if fp64:
             raise ValueError("the compiled reference has no FP64-output mode")
@@ -153,7 +159,7 @@ def join(a, b, m, profile_type="1NN_INDEX", max_tile_size=1 << 17,
             _p(a), na, _p(b) if has_b else None, nb, m, pt, max_tile_size,
             int(computing_rows), int(computing_cols), int(keep_rows_separate),
             int(is_aligned), global_start_row, global_start_col, threshold,
-            num_workers, _p(ca), _p(ia, _ip), _p(cb), _p(ib, _ip))
+            num_workers, PRECISIONS[precision], _p(ca), _p(ia, _ip), _p(cb), _p(ib, _ip))
     else:
         args = _OraArgs(m, max_tile_size, pt, int(has_b), int(computing_rows),
                         int(computing_cols), int(keep_rows_separate), int(is_aligned),

@@ -22,8 +22,8 @@ extern "C" int ref_join(const double *a, int64_t na, const double *b,
                         int64_t nb, int64_t m, int ptype, int64_t max_tile,
                         int rows, int cols, int keep_rows, int aligned,
                         int64_t grow, int64_t gcol, double thr, int nthreads,
-                        double *corr_a, int64_t *idx_a, double *corr_b,
-                        int64_t *idx_b) {
+                        int precision, double *corr_a, int64_t *idx_a,
+                        double *corr_b, int64_t *idx_b) {
   try {
     SCAMP::SCAMPArgs args;
     args.timeseries_a.assign(a, a + na);
@@ -38,7 +38,9 @@ extern "C" int ref_join(const double *a, int64_t na, const double *b,
     args.distributed_start_row = grow;
     args.distributed_start_col = gcol;
     args.distance_threshold = thr;
-    args.precision_type = SCAMP::PRECISION_DOUBLE;
+    // SCAMPPrecisionType values (src/common.h:53-58); the CPU path runs
+    // DOUBLE and ULTRA (src/cpu_kernels.cpp:201-252)
+    args.precision_type = static_cast<SCAMP::SCAMPPrecisionType>(precision);
     args.profile_type = static_cast<SCAMP::SCAMPProfileType>(ptype);
     args.computing_rows = rows != 0;
     args.computing_columns = cols != 0;

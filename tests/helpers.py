@@ -119,3 +119,30 @@ def euclidean(corr, m):
     d = np.sqrt(np.maximum(2 * m * (1 - c), 0))
     d[c < -1] = np.nan
     return d
+
+
+def reduced_precision_errors(a, b, m, corr, idx, ref_corr, ref_idx):
+    """Errors of a reduced-precision 1NN_INDEX profile (a's windows matched in
+    b; b None: a itself) against FP64 reference values: max |dcorr| over the
+    valid entries, the raw index agreement, the agreement when ties under the
+    reference's rule (test/test_common.py:3-17: the two candidates' FP64
+    correlations within 1e-9 or equal after FP32 rounding) count as agreeing,
+    the number of non-tie disagreements, and the largest exact FP64 gap between
+    the reference's match and the reduced form's at a disagreement."""
+    bb = a if b is None else b
+    corr = np.asarray(corr, dtype=np.float64)
+    ref_corr = np.asarray(ref_corr, dtype=np.float64)
+    ok = ref_corr > -2
+    n = int(ok.sum())
+    dmax = float(np.max(np.abs(corr[ok] - ref_corr[ok]), initial=0.0))
+    dis = np.nonzero(ok & (np.asarray(idx) != np.asarray(ref_idx)))[0]
+    gap, non_tie = 0.0, 0
+    if dis.size:
+        er = exact_corr(a, bb, m, dis, np.asarray(ref_idx)[dis])
+        eg = exact_corr(a, bb, m, dis, np.asarray(idx)[dis])
+        gap = float(np.max(er - eg))
+        tie = (np.abs(er - eg) <= 1e-9) | (er.astype(np.float32) == eg.astype(np.float32))
+        non_tie = int(np.count_nonzero(~tie))
+    return {"entries": n, "max_dcorr": dmax, "raw_agreement": 1 - dis.size / max(n, 1),
+            "agreement": 1 - non_tie / max(n, 1), "non_tie": non_tie,
+            "disagreements": int(dis.size), "max_gap": gap}

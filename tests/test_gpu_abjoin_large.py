@@ -67,6 +67,10 @@ def direct_corr(q, y, j):
 def joins(a, b, precision):
     nn_a, nn_b = C.join(a, b, C.make_params(M, "1NN_INDEX", precision=precision, has_b=True,
                                             computing_rows=True, keep_rows_separate=True))
+    ran = C.context(0).last_precision()
+    print(f"1NN_INDEX {precision}: ran in {ran[0]}, audit failures {ran[1]}", flush=True)
+    # the mixed results below must come from the FP32 sweep itself
+    assert ran == (precision, 0), ran
     s_a, s_b = C.join(a, b, C.make_params(M, "SUM_THRESH", precision=precision, has_b=True,
                                           computing_rows=True, keep_rows_separate=True,
                                           threshold=THR))
