@@ -56,6 +56,11 @@ template <typename DT>
 struct Slot {
   DT df, dg, in;   // column update terms and inverse norm
   DT t;            // 1NN: threshold of the column's current best
+  // Two-stage 1NN filter (see Sweep::kTwoStage): mt = min(block row bound,
+  // column threshold), the one-compare first stage; the FP64 form keeps its
+  // column threshold as the integer image F (enc_thr) instead of t.
+  DT mt;
+  int32_t F;
   double acc;      // SUM: partial sum of the column, travelling with it
 };
 
@@ -63,8 +68,41 @@ struct Slot {
 template <typename DT>
 struct Terms {
   DT df, dg, in;
-  DT t;  // 1NN: threshold of the row's / column's current best
+  DT t;       // 1NN: threshold of the row's / column's current best
 };
+template <>
+struct Terms<double> {
+  double df, dg, in;
+  double t;
+  int32_t F;  // two-stage filter: enc_thr of the column's threshold
+};
+
+// Integer image of an FP64 threshold t for the two-stage filter: for c >= 0,
+// c >= t implies hi32(c) >= enc_thr(t) (the high word of a non-negative
+// double orders like the double and truncates it downwards); a negative
+// threshold passes every correlation (kNegEnc decodes to -1.0625 with a
+// zero low word); a window that can never match gets kNanEnc, which no
+// correlation's high word reaches and which decodes to NaN.
+constexpr int32_t kNegEnc = (int32_t)0xBFF10000u;
+constexpr int32_t kNanEnc = 0x7FFFFFFF;
+__device__ __forceinline__ int32_t enc_thr(double t) {
+  return t >= 0.0 ? __double2hiint(t) : kNegEnc;
+}
+// The FP32 two-stage filter orders thresholds by their bit patterns (one
+// integer min, no NaN handling): thresholds enter with every negative value
+// replaced by -2, below any correlation, so that the bit patterns of the
+// values in play (-2 and non-negative ones) order like the values.
+__device__ __forceinline__ float canon_thr(float t) { return t < 0.0f ? -2.0f : t; }
+__device__ __forceinline__ float min_thr(float a, float b) {
+  const int32_t x = __float_as_int(a), y = __float_as_int(b);
+  return __int_as_float(x < y ? x : y);
+}
+// The first-stage FP64 threshold: the double whose high word is
+// min(row bound, column image) and whose low word is 0 (so it never exceeds
+// either threshold it stands for).
+__device__ __forceinline__ double mt_of(int32_t tlo, int32_t f) {
+  return __hiloint2double(tlo < f ? tlo : f, 0);
+}
 
 template <typename DT>
 __device__ __forceinline__ DT nan_of();
@@ -82,6 +120,9 @@ __device__ __forceinline__ float nan_of<float>() {
 __device__ __forceinline__ float shl1(float v, float edge) {
   return __int_as_float(__builtin_amdgcn_update_dpp(
       __float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ int32_t shl1(int32_t v, int32_t edge) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xf, 0xf, false);
 }
 __device__ __forceinline__ double shl1(double v, double edge) {
   const long long x = __double_as_longlong(v), e = __double_as_longlong(edge);
@@ -131,6 +172,18 @@ struct Sweep {
   // sweep_kernel this measured 4.4 % (FP64) and 7.5 % (FP32) faster at
   // n = 2^22, m = 1024 than one cycle per block.
   static constexpr int kRows = KIND == kNN ? 2 * K : K;
+  // Two-stage 1NN filter (rows and columns, long chunks): every pair is
+  // first compared once with mt = min(T_lo, column threshold), T_lo the
+  // lowest row threshold of the block; only rows where some pair passes run
+  // the exact row and column tests.  With converged thresholds a few percent
+  // of rows pass the first stage, so a pair costs one compare instead of
+  // two.  The FP64 form keeps each column's threshold as an integer image
+  // (one register, one DPP move) next to the double mt.
+  static constexpr bool kTwoStage = KIND == kNN && !FRESH && COLS && ROWS;
+  static constexpr bool kIntCol = kTwoStage && sizeof(DT) == 8;
+  // the block's row bound: FP32 the threshold itself, FP64 its enc_thr image
+  DT tlo_f = (DT)0;
+  int32_t tlo_i = 0;
   DT cov[K];
   DT hi[K];
   uint64_t *s_kq = nullptr;  // ALL_NEIGHBORS: per-lane candidate slots (LDS)
@@ -176,10 +229,26 @@ struct Sweep {
     s.df = ld<DT>(a.dfX, off);
     s.dg = ld<DT>(a.dgX, off);
     s.in = ld<DT>(a.inX, off);
-    if (kThr && COLS)
-      s.t = Thresh<DT>::from_key_hi(
+    if (kThr && COLS) {
+      const DT t = Thresh<DT>::from_key_hi(
           ld<uint32_t>(a.keyX, static_cast<uint32_t>(col) * 8u + 4u));
+      if constexpr (kIntCol)
+        s.F = s.in == s.in ? enc_thr(t) : kNanEnc;
+      else if constexpr (kTwoStage)
+        s.t = canon_thr(t);
+      else
+        s.t = t;
+    }
     if (KIND == kSum) s.acc = 0.0;
+  }
+
+  // Two-stage filter: the first-stage threshold of a slot from the block's
+  // row bound and the column's threshold.
+  __device__ __forceinline__ void refresh_mt(Slot<DT> &s) const {
+    if constexpr (kIntCol)
+      s.mt = mt_of(tlo_i, s.F);
+    else
+      s.mt = min_thr(tlo_f, s.t);
   }
 
   // Terms staged a block ahead: lane l fetches row ib+l (the rows of the
@@ -210,7 +279,7 @@ struct Sweep {
       p.ckey = ld<uint32_t>(a.keyX, static_cast<uint32_t>(ci) * 8u + 4u);
   }
   __device__ __forceinline__ void publish(const Pre &p, Terms<DT> *s_row,
-                                          Terms<DT> *s_col) const {
+                                          Terms<DT> *s_col) {
     Terms<DT> r, c;
     r.df = p.rdf;
     r.dg = p.rdg;
@@ -220,8 +289,34 @@ struct Sweep {
     c.dg = p.cdg;
     c.in = p.cin;
     c.t = (kThr && COLS) ? Thresh<DT>::from_key_hi(p.ckey) : (DT)0;
+    if constexpr (kIntCol) {
+      r.F = 0;
+      c.F = c.in == c.in ? enc_thr(c.t) : kNanEnc;
+    } else if constexpr (kTwoStage) {
+      r.t = canon_thr(r.t);
+      c.t = canon_thr(c.t);
+    }
     s_row[lane] = r;
     s_col[lane] = c;
+    if constexpr (kTwoStage) {
+      // T_lo: the lowest threshold of the block's rows (lanes < kRows stage
+      // them; rows that can never match do not count)
+      const bool mine = lane < kRows && r.in == r.in;
+      if constexpr (kIntCol) {
+        int32_t v = mine ? enc_thr(r.t) : kNanEnc;
+#pragma unroll
+        for (int off = kRows / 2; off > 0; off >>= 1) {
+          const int32_t o = __shfl_xor(v, off);
+          v = o < v ? o : v;
+        }
+        tlo_i = __builtin_amdgcn_readfirstlane(v);
+      } else {
+        float v = mine ? (float)r.t : __builtin_inff();
+#pragma unroll
+        for (int off = kRows / 2; off > 0; off >>= 1) v = min_thr(v, __shfl_xor(v, off));
+        tlo_f = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+      }
+    }
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -476,13 +571,21 @@ struct Sweep {
   }
   // Row i's pairs that beat their column's best, into the columns' entries
   // (FRESH: into the block's column table).
+  __device__ __forceinline__ bool col_passes(DT c, const Slot<DT> &s) const {
+    // the FP64 two-stage form tests the integer image (conservative; the
+    // NaN of an invalid diagonal passes it and is caught here)
+    if constexpr (kIntCol)
+      return c == c && __double2hiint(c) >= s.F;
+    else
+      return c >= s.t;
+  }
   __device__ __forceinline__ void nn_col_update(int r, int rr, const DT (&c)[K],
                                                 int64_t i) {
     if (kColTable) ct_dirty = true;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       Slot<DT> &s = S[(r + k) % K];
-      if (c[k] >= s.t) {
+      if (col_passes(c[k], s)) {
         const float f = (float)c[k];
         const uint64_t key = make_key(f, static_cast<uint32_t>(i));
         if (kColTable)
@@ -496,7 +599,15 @@ struct Sweep {
         // atomic's return value); the tighter threshold travels with the
         // column to the next lanes
         const DT tf = Thresh<DT>::from_corr(f);
-        s.t = s.t > tf ? s.t : tf;
+        if constexpr (kIntCol) {
+          const int32_t ef = enc_thr(tf);
+          s.F = s.F > ef ? s.F : ef;
+        } else if constexpr (kTwoStage) {
+          const DT tc = canon_thr(tf);
+          s.t = s.t > tc ? s.t : tc;
+        } else {
+          s.t = s.t > tf ? s.t : tf;
+        }
       }
     }
   }
@@ -663,6 +774,23 @@ struct Sweep {
       }
       if (ROWS && hit_r) nn_row_update(c, i, rr, trow);
       if (COLS && hit_c) nn_col_update(r, rr, c, i);
+    } else if (kTwoStage) {
+      uint64_t hit1 = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        hit1 |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].mt);
+      if (hit1) {
+        uint64_t hit = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
+          hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
+        }
+        if (hit) {
+          nn_row_update(c, i, rr, trow);
+          nn_col_update(r, rr, c, i);
+        }
+      }
     } else if (kThr) {
       // One compare per test written straight into a wave mask (keeps the
       // compiler from merging the two tests into a NaN-canonicalising min).
@@ -735,7 +863,11 @@ struct Sweep {
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
     s.in = shl1(s.in, e.in);
-    if (kThr && COLS) s.t = shl1(s.t, e.t);
+    if constexpr (kIntCol)
+      s.F = shl1(s.F, e.F);
+    else if (kThr && COLS)
+      s.t = shl1(s.t, e.t);
+    if constexpr (kTwoStage) refresh_mt(s);
   }
 
   // Flushes what a lane still holds when its segment ends after row i.
@@ -796,6 +928,10 @@ struct Sweep {
                                              Terms<DT> *s_row, Terms<DT> *s_col,
                                              bool first) {
     publish(pre, s_row, s_col);
+    if constexpr (kTwoStage) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) refresh_mt(S[k]);
+    }
     prefetch(pre, i_b + kRows);
     // keep the look-ahead loads where they are issued: the scheduler would
     // otherwise sink them next to their first use a block later
