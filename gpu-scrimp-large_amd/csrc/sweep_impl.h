@@ -170,8 +170,16 @@ struct Sweep {
   // which halves the per-block work (staging, threshold decoding,
   // loop-carried register copies); together with the occupancy targets on
   // sweep_kernel this measured 4.4 % (FP64) and 7.5 % (FP32) faster at
-  // n = 2^22, m = 1024 than one cycle per block.
-  static constexpr int kRows = KIND == kNN ? 2 * K : K;
+  // n = 2^22, m = 1024 than one cycle per block.  The two-stage kernels
+  // (below) take longer blocks still, which spreads the block's staging and
+  // row-bound reduction over more rows at the price of a looser row bound
+  // and more code.  Measured in one process at n = 2^22, m = 1024 (16 / 32 /
+  // 64 rows): FP64 1719 / 1663 / 1667 ms, FP32 1423 / 1386 / 1329 ms.  A
+  // block is at most one row per lane of the staging arrays.
+  static constexpr int kRows =
+      KIND == kNN ? (!FRESH && COLS && ROWS ? (sizeof(DT) == 4 ? 8 * K : 4 * K) : 2 * K)
+                  : K;
+  static_assert(kRows <= kWave, "one staged row and column per lane");
   // Two-stage 1NN filter (rows and columns, long chunks): every pair is
   // first compared once with mt = min(T_lo, column threshold), T_lo the
   // lowest row threshold of the block; only rows where some pair passes run
