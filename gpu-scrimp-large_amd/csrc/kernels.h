@@ -120,27 +120,18 @@ hipError_t launch_knn_to_matches(uint64_t *lists, int64_t n, int k, uint64_t ini
 // ---- the diagonal sweep (sweep_kernel.hip) --------------------------------
 enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3, kKnn = 4 };
 
-// One work item of the sweep: rows [rs, re) of one 512-diagonal band, then
-// rows [rs2, re2) of it (empty when re2 <= rs2).
-struct SweepItem {
-  int64_t rs, re, rs2, re2;  // rs, rs2 are multiples of 8
-  int64_t seed, seed2;       // FP64 form: offsets in `seeds` of the band's 512
-                             // exact covariances at rows rs and rs2
-  int32_t band;              // band b: diagonals d_base + 512 b + [0, 512)
-  int32_t series;            // batched joins: the series
-};
-
 struct SweepArgs {
   // column side X / row side Y update terms and inverse norms (FP64 or FP32,
   // padded by kColumnPad entries)
   const void *dfX, *dgX, *inX;
   const void *dfY, *dgY, *inY;
-  // clean series and means (FP32 form: in-kernel FP64 re-anchoring)
+  // clean series and means (the in-kernel FP64 seeds and re-anchors)
   const double *TX, *TY, *muX, *muY;
-  // work items, taken in order through `counter`, and their FP64 seeds
-  const SweepItem *items;
+  // work items, taken in order through `counter` and decoded in the kernel
+  // (plan_item): nitems = bands x nbatch x nchunks; chunk_rows is the row
+  // length R of a chunk when nchunks > 1
   int64_t nitems;
-  const double *seeds;
+  int64_t nchunks, chunk_rows;
   int64_t nX, nY;        // subsequence counts of X and Y
   int64_t d_lo, d_hi;    // diagonal range of this pass
   int64_t d_base;        // d_lo rounded down to a multiple of 8: band b

@@ -226,7 +226,8 @@ struct Pass {
 // Row chunks up to this length run the 1NN sweep's short-chunk variant.
 constexpr int64_t kFreshMaxRows = 16384;
 
-// Runs one pass (one diagonal range of one orientation).
+// Runs one pass (one diagonal range of one orientation).  sweep_ms: adds
+// the sweep's HIP-event time (null: no events, no wait for the sweep).
 void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
               bool fp32, SweepKind kind, uint32_t *mat, int64_t mat_w,
               int64_t rpc, int64_t cpc, double thr, int64_t anchor, int knn_k,
@@ -244,25 +245,25 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
   const int64_t d_base = p.d_lo - p.d_lo % kDiagsPerLane;
   const int64_t nbands = (p.d_hi - d_base + kBand - 1) / kBand;
 
-  // Work items.  Band b spans rows [0, rows0) (rows with at least one valid
-  // diagonal); all of its diagonals are valid below rfull.  Its rows are cut
-  // at r0 + j * R: r0 is a hashed multiple of 8 below rfull, so that items
+  // Work items (decoded by the sweep itself, plan_item in sweep_impl.h).
+  // Band b spans rows [0, rows0) (rows with at least one valid diagonal);
+  // all of its diagonals are valid below rfull.  Its rows are cut at
+  // r0 + j * R: r0 is a hashed multiple of 8 below rfull, so that items
   // running at the same time work on different rows' profile entries; R is
   // unbounded when the pass has enough bands to fill the GPU, otherwise
   // short enough that the items do, and never below max(512, 4m) rows so the
-  // seeds stay cheap next to the rows they start.
-  std::vector<int64_t> rows0(nbands), r0(nbands);
-  int64_t total_rows = 0;
-  for (int64_t b = 0; b < nbands; ++b) {
+  // segment seeds stay cheap next to the rows they start.
+  auto band_rows = [&](int64_t b, int64_t *rfull) {
     const int64_t D0 = d_base + b * kBand;
     const int64_t dfirst = std::max(D0, p.d_lo);
     const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
-    rows0[b] = std::min(nY, nX - dfirst);
-    const int64_t rfull = std::min(nY, nX - dlast);
-    const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
-    r0[b] = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
-                      (uint64_t)slots) * kDiagsPerLane;
-    total_rows += rows0[b];
+    *rfull = std::min(nY, nX - dlast);
+    return std::min(nY, nX - dfirst);
+  };
+  int64_t total_rows = 0;
+  for (int64_t b = 0; b < nbands; ++b) {
+    int64_t rf;
+    total_rows += std::max<int64_t>(0, band_rows(b, &rf));
   }
   // (a batch fills the GPU with its series' bands alone once there are
   // enough of them; the chunk length itself depends on one series only)
@@ -273,80 +274,31 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
     R = std::max(floor_rows, (total_rows + 4 * wave_slots - 1) / (4 * wave_slots));
     R = (R + 15) / 16 * 16;
   }
-  // Items in band order (of every series), each band's from row r0 up and
-  // then from row 0: the low diagonals, where a series' best matches mostly
-  // lie, come first, so the profile thresholds that let most rows skip their
-  // atomic merges rise early; the triangle's short high bands form the tail.
-  // Unchunked, one item holds both of a band's segments, [r0, rows0) and
-  // then [0, r0), so concurrent bands stay at unrelated rows.
-  std::vector<SweepItem> items;
-  std::vector<int64_t> cuts;
   const bool chunked = R != std::numeric_limits<int64_t>::max();
-  for (int64_t b = 0; b < nbands; ++b) {
-    if (!chunked) {
-      for (int64_t q = 0; q < nbatch; ++q)
-        items.push_back(SweepItem{r0[b], rows0[b], 0, r0[b], 0, 0, (int32_t)b,
-                                  (int32_t)q});
-      continue;
+  int64_t nchunks = 1;
+  if (chunked) {
+    // the largest chunk count of any band (the kernel skips the items a
+    // shorter band does not have); the same cuts as plan_item
+    for (int64_t b = 0; b < nbands; ++b) {
+      int64_t rfull;
+      const int64_t rows0 = band_rows(b, &rfull);
+      if (rows0 <= 0 || rfull <= 0) continue;
+      const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
+      const int64_t r0 = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
+                                   (uint64_t)slots) * kDiagsPerLane;
+      int64_t first = r0 % R;
+      if (first == 0) first = R;
+      const int64_t n = (first < rows0 ? (rows0 - 1 - first) / R + 1 : 0) + 1;
+      nchunks = std::max(nchunks, n);
     }
-    // boundaries 0 < r0 + j R < rows0 (ascending), rows0
-    cuts.assign(1, 0);
-    int64_t c = r0[b];
-    if (R < rows0[b])
-      while (c - R > 0) c -= R;
-    for (; c < rows0[b]; c = (R > rows0[b] - c) ? rows0[b] : c + R)
-      if (c > 0) cuts.push_back(c);
-    cuts.push_back(rows0[b]);
-    // r0 is 0 or a boundary: the items from it upwards, then those below it
-    const size_t n_it = cuts.size() - 1;
-    const size_t first = std::lower_bound(cuts.begin(), cuts.end(), r0[b]) - cuts.begin();
-    for (int64_t q = 0; q < nbatch; ++q)
-      for (size_t j = 0; j < n_it; ++j) {
-        const size_t k = (first + j) % n_it;
-        items.push_back(SweepItem{cuts[k], cuts[k + 1], 0, 0, 0, 0, (int32_t)b,
-                                  (int32_t)q});
-      }
   }
-  const int64_t nitems = (int64_t)items.size();
-  if (nitems >= (int64_t)0xFFFFFFFFu)
+  const int64_t nitems = nbands * nbatch * nchunks;
+  if (nitems >= (int64_t)0xFFFFFFFFu - (int64_t)ctx->num_cus * 16)
     throw HipFailure{hipErrorInvalidValue, "too many sweep work items"};
-  double *seeds = nullptr;
-  if (!fp32) {
-    // the exact covariance of the band's diagonals at the first row of every
-    // segment
-    std::vector<SeedTask> tasks;
-    int64_t nseeds = 0;
-    auto seed_tasks = [&](const SweepItem &it, int64_t row) {
-      const int64_t off = nseeds;
-      nseeds += kBand;
-      const int64_t D0 = d_base + it.band * kBand;
-      const int64_t dfirst = std::max(D0, p.d_lo);
-      const int64_t dlast = std::min(D0 + kBand, p.d_hi) - 1;
-      const int64_t cnt = dlast - dfirst + 1;
-      const int64_t roff = (int64_t)it.series * stride;
-      for (int64_t j = 0; j < cnt; j += 256)
-        tasks.push_back(SeedTask{roff + row, dfirst + j, off + (dfirst - D0) + j,
-                                 (int32_t)std::min<int64_t>(256, cnt - j), 0});
-      return off;
-    };
-    for (SweepItem &it : items) {
-      it.seed = seed_tasks(it, it.rs);
-      if (it.re2 > it.rs2) it.seed2 = seed_tasks(it, it.rs2);
-    }
-    seeds = ar.alloc<double>(nseeds);
-    SeedTask *d_tasks = ar.alloc<SeedTask>((int64_t)tasks.size());
-    HIP_CHECK(hipMemcpyAsync(d_tasks, tasks.data(),
-                             sizeof(SeedTask) * tasks.size(),
-                             hipMemcpyHostToDevice, s));
-    HIP_CHECK(launch_seeds(d_tasks, (int64_t)tasks.size(), p.X->clean,
-                           p.X->mean, p.Y->clean, p.Y->mean, m, seeds, s));
-  }
-  SweepItem *d_items = ar.alloc<SweepItem>(nitems);
-  HIP_CHECK(hipMemcpyAsync(d_items, items.data(), sizeof(SweepItem) * nitems,
-                           hipMemcpyHostToDevice, s));
 
-  unsigned int *counter = ar.alloc<unsigned int>(1);
-  HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
+  // the work counter, cleared by a fill kernel (no memset node)
+  uint64_t *counter = ar.alloc<uint64_t>(1);
+  HIP_CHECK(launch_fill_u64(counter, 1, 0, s));
 
   SweepArgs a{};
   if (fp32) {
@@ -357,24 +309,28 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
     a.dfY = p.Y->df; a.dgY = p.Y->dg; a.inY = p.Y->inorm;
   }
   a.TX = p.X->clean; a.TY = p.Y->clean; a.muX = p.X->mean; a.muY = p.Y->mean;
-  a.items = d_items; a.nitems = nitems; a.seeds = seeds;
+  a.nitems = nitems; a.nchunks = nchunks; a.chunk_rows = chunked ? R : 0;
   a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi; a.d_base = d_base;
   a.m = m; a.anchor = anchor;
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
   a.knnX = p.knnX; a.knnY = p.knnY; a.knn_k = knn_k;
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
   a.mat_mirror = p.X == p.Y && rpc == cpc;
-  a.counter = counter;
+  a.counter = reinterpret_cast<unsigned int *>(counter);
   a.nbatch = nbatch;
   a.batch_stride = stride;
 
   const int grid = (int)std::min<int64_t>(nitems, (int64_t)ctx->num_cus * 16);
-  Event e0, e1;
-  HIP_CHECK(hipEventRecord(e0.e, s));
   // Short row chunks start with the profile far below its final values, so
   // many of their rows merge: the 1NN sweep then runs its variant with
   // block-level merge tables (the long-chunk kernel keeps fewer registers).
   const bool fresh = chunked && R <= kFreshMaxRows;
+  if (sweep_ms == nullptr) {
+    HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, fresh));
+    return;
+  }
+  Event e0, e1;
+  HIP_CHECK(hipEventRecord(e0.e, s));
   HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, fresh));
   HIP_CHECK(hipEventRecord(e1.e, s));
   HIP_CHECK(hipEventSynchronize(e1.e));

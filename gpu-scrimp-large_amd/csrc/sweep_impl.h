@@ -50,7 +50,11 @@ namespace scamp_amd {
 namespace sweep {
 
 constexpr int K = kDiagsPerLane;
-constexpr int kAnchorChunk = 256;  // u-samples per LDS stage of an anchor
+// u-samples per LDS stage of a seed / re-anchor: the FP32 form has its own
+// staging; the FP64 form stages through its row and column term arrays
+// (idle at a segment start, the only place it seeds), which hold 640 doubles
+constexpr int kAnchorChunk = 256;
+constexpr int kAnchorChunk64 = 64;
 
 template <typename DT>
 struct Slot {
@@ -329,18 +333,23 @@ struct Sweep {
   }
 
   // ---------------------------------------------------------------- seeds
-  // FP32 form: exact FP64 covariance of this lane's 8 diagonals at row i,
-  //   cov_k = sum_t X[i+d_k+t]*yc_t - muX[i+d_k]*sum_t yc_t,
-  //   yc_t = Y[i+t] - muY[i]
-  // The wave stages the X samples of its 512 diagonals and the centred Y
-  // window through LDS in chunks (coalesced loads, one pass), then each lane
-  // slides a register window over them so that each X sample feeds 8 FMAs.
+  // Exact-in-FP64 covariance of this lane's 8 diagonals at row i: the seed
+  // of every segment (both forms) and the FP32 form's periodic re-anchor,
+  //   cov_k = sum_t (X[i+d_k+t] - ref) * yc_t - (muX[i+d_k] - ref) * sum_t yc_t,
+  //   yc_t = Y[i+t] - muY[i],  ref = muX[i + D0]
+  // (sum_t (X - muX) yc = sum_t (X - ref) yc - (muX - ref) sum_t yc; the
+  // reference level keeps the products near the series' local scale, so
+  // the FP64 accumulation does not cancel a large offset).  The wave stages
+  // the X samples of its 512 diagonals and the centred Y window through LDS
+  // in chunks (coalesced loads, one pass), then each lane slides a register
+  // window over them so that each X sample feeds 8 FMAs: 8m FMAs per lane.
   __device__ __forceinline__ void anchor(int64_t i, double *s_ax,
                                          double *s_ay) {
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
     const double my = a.muY[i];
+    const double ref = a.muX[i + D0];
     const int64_t m = a.m;
     const int64_t uend = m + K - 1;  // u = t + k runs over [0, m + K - 1)
     const int xoff = lane * K;       // d0 - D0
@@ -348,16 +357,17 @@ struct Sweep {
     double yw[2 * K - 1];            // yc_t for t = u0-K+1 .. u0+K-1
 #pragma unroll
     for (int q = 0; q < 2 * K - 1; ++q) yw[q] = 0.0;
-    for (int64_t c0 = 0; c0 < uend; c0 += kAnchorChunk) {
+    constexpr int kChunk = sizeof(DT) == 8 ? kAnchorChunk64 : kAnchorChunk;
+    for (int64_t c0 = 0; c0 < uend; c0 += kChunk) {
       __builtin_amdgcn_wave_barrier();
       const double *xs = a.TX + i + D0 + c0;
-      for (int j = lane; j < kBand + kAnchorChunk; j += kWave) s_ax[j] = xs[j];
-      for (int j = lane; j < kAnchorChunk; j += kWave) {
+      for (int j = lane; j < kBand + kChunk; j += kWave) s_ax[j] = xs[j] - ref;
+      for (int j = lane; j < kChunk; j += kWave) {
         const int64_t t = c0 + j;
         s_ay[j] = t < m ? a.TY[i + t] - my : 0.0;
       }
       __builtin_amdgcn_wave_barrier();
-      const int64_t cend = uend - c0 < kAnchorChunk ? uend - c0 : kAnchorChunk;
+      const int64_t cend = uend - c0 < kChunk ? uend - c0 : kChunk;
       for (int u = 0; u < cend; u += K) {
 #pragma unroll
         for (int q = 0; q < K - 1; ++q) yw[q] = yw[q + K];
@@ -378,17 +388,21 @@ struct Sweep {
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // the FP32 row and column tables share s_ay and s_ax
+    // the FRESH row and column tables share s_ay and s_ax
     if (kTable && lane < kRows) s_rt[lane] = 0;
     if (kColTable)
       for (int q = lane; q < K * kColStride; q += kWave) s_ct[q] = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
-      const double v = acc[k] - a.muX[i + d] * sy;
-      const DT h = (DT)v;
-      hi[k] = diag_valid(d) ? h : nan_of<DT>();
-      cov[k] = (DT)(v - (double)h);
+      const double v = acc[k] - (a.muX[i + d] - ref) * sy;
+      if constexpr (kComp) {
+        const DT h = (DT)v;
+        hi[k] = diag_valid(d) ? h : nan_of<DT>();
+        cov[k] = (DT)(v - (double)h);
+      } else {
+        cov[k] = diag_valid(d) ? (DT)v : nan_of<DT>();
+      }
     }
   }
 
@@ -401,15 +415,6 @@ struct Sweep {
       const DT s = hi[k] + cov[k];
       cov[k] = cov[k] - (s - hi[k]);
       hi[k] = s;
-    }
-  }
-
-  // FP64 form: seeds computed by the exact-summation seed kernel.
-  __device__ __forceinline__ void seed_from(const double *src) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double v = src[k];
-      cov[k] = diag_valid(d0 + k) ? (DT)v : nan_of<DT>();
     }
   }
 
@@ -1032,34 +1037,102 @@ constexpr int sweep_waves_per_eu() {
   return 1;
 }
 
+// The rows one work item sweeps: [rs, re) of a band, then [rs2, re2)
+// (empty when re2 <= rs2).
+struct ItemRows {
+  int64_t band, series, rs, re, rs2, re2;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+// Work item idx of a pass, decoded on the device (no host-built list).
+// Items run band-major (band b of every series, then band b + 1), so the low
+// diagonals, where a series' best matches mostly lie, go first.  A band
+// spans rows [0, rows0) and starts at a hashed row r0 (a multiple of 8 below
+// rfull), so that bands in flight sit at unrelated rows' profile entries:
+//  * unchunked (nchunks == 1): one item, rows [r0, rows0) then [0, r0);
+//  * chunked: the rows are cut at r0 + j R, and the band's item j is its
+//    j-th chunk counting from the one that starts at r0 (wrapping round to
+//    row 0); items beyond the band's chunk count are empty.
+// Returns false for an empty item.
+__device__ __forceinline__ bool plan_item(const SweepArgs &a, unsigned int idx,
+                                          ItemRows &g) {
+  const uint64_t per_band = (uint64_t)a.nbatch * (uint64_t)a.nchunks;
+  g.band = (int64_t)(idx / per_band);
+  const int64_t rem = (int64_t)(idx - (uint64_t)g.band * per_band);
+  g.series = rem / a.nchunks;
+  const int64_t j = rem - g.series * a.nchunks;
+  const int64_t D0 = a.d_base + g.band * kBand;
+  const int64_t dfirst = D0 > a.d_lo ? D0 : a.d_lo;
+  const int64_t dlast = (D0 + kBand < a.d_hi ? D0 + kBand : a.d_hi) - 1;
+  const int64_t rows0 = a.nY < a.nX - dfirst ? a.nY : a.nX - dfirst;
+  const int64_t rfull = a.nY < a.nX - dlast ? a.nY : a.nX - dlast;
+  if (rows0 <= 0 || rfull <= 0) return false;
+  const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
+  const int64_t r0 =
+      (int64_t)(mix64((uint64_t)g.band + 0x9e3779b97f4a7c15ULL) % (uint64_t)slots) *
+      kDiagsPerLane;
+  if (a.nchunks == 1) {
+    g.rs = r0;
+    g.re = rows0;
+    g.rs2 = 0;
+    g.re2 = r0;
+    return true;
+  }
+  const int64_t R = a.chunk_rows;
+  int64_t first = r0 % R;  // the lowest cut above row 0
+  if (first == 0) first = R;
+  const int64_t cuts = first < rows0 ? (rows0 - 1 - first) / R + 1 : 0;
+  const int64_t n = cuts + 1;
+  if (j >= n) return false;
+  const int64_t k_r0 = r0 == 0 ? 0 : (r0 - first) / R + 1;
+  const int64_t k = (k_r0 + j) % n;
+  g.rs = k == 0 ? 0 : first + (k - 1) * R;
+  g.re = k == n - 1 ? rows0 : first + k * R;
+  g.rs2 = 0;
+  g.re2 = 0;
+  return true;
+}
+
 template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH>
 __global__ __launch_bounds__(kWave)
 __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<DT, KIND>())))
 void sweep_kernel(SweepArgs a) {
-  __shared__ Terms<DT> s_row[kWave];
-  __shared__ Terms<DT> s_col[kWave];
-  // FP32 form only: anchor staging (unreferenced, hence unallocated, in FP64)
-  __shared__ double s_ax[sizeof(DT) == 4 ? kBand + kAnchorChunk : 1];
-  __shared__ double s_ay[sizeof(DT) == 4 ? kAnchorChunk : 1];
+  __shared__ Terms<DT> s_terms[2 * kWave];  // staged row / column terms
+  Terms<DT> *const s_row = s_terms;
+  Terms<DT> *const s_col = s_terms + kWave;
+  // FP64 staging of segment seeds: the term arrays (no block is in flight at
+  // a segment start); FP32 (which also re-anchors mid-segment): its own
+  constexpr bool kF32 = sizeof(DT) == 4;
+  static_assert(kF32 || (kBand + 2 * kAnchorChunk64) * sizeof(double) <= sizeof(s_terms),
+                "the FP64 seed staging fits in the term arrays");
+  __shared__ double s_anchor_own[kF32 ? kBand + 2 * kAnchorChunk : 1];
+  double *const s_ax = kF32 ? s_anchor_own : reinterpret_cast<double *>(s_terms);
+  double *const s_ay = kF32 ? s_anchor_own + kBand + kAnchorChunk
+                            : reinterpret_cast<double *>(s_terms) + kBand + kAnchorChunk64;
   __shared__ uint64_t s_kq[KIND == kKnn ? kBand : 1];
   // FRESH: best key of each row of the current block, and of each column
-  // the block crosses (zero = none).  The FP32 form keeps both tables in the
+  // the block crosses (zero = none).  The FP32 form keeps both tables in its
   // anchor staging, which is only in use between blocks (anchor() clears
   // them again when it is done).
   constexpr bool kTable = KIND == kNN && FRESH;
-  __shared__ uint64_t s_rt_own[kTable && sizeof(DT) == 8 ? 2 * kDiagsPerLane : 1];
   using SW = Sweep<DT, KIND, COLS, ROWS, FRESH>;
-  __shared__ uint64_t s_ct_own[SW::kColTable && sizeof(DT) == 8
-                                   ? kDiagsPerLane * SW::kColStride
-                                   : 1];
+  __shared__ uint64_t s_rt_own[kTable && !kF32 ? SW::kRows : 1];
+  __shared__ uint64_t s_ct_own[SW::kColTable && !kF32 ? kDiagsPerLane * SW::kColStride : 1];
   static_assert(kDiagsPerLane * SW::kColStride <= kBand + kAnchorChunk,
                 "the FP32 column table fits in the anchor staging");
-  uint64_t *const s_ct =
-      sizeof(DT) == 4 ? reinterpret_cast<uint64_t *>(s_ax) : s_ct_own;
-  uint64_t *const s_rt =
-      sizeof(DT) == 4 ? reinterpret_cast<uint64_t *>(s_ay) : s_rt_own;
+  static_assert(SW::kRows <= kAnchorChunk, "the FP32 row table fits in the anchor staging");
+  uint64_t *const s_ct = kF32 ? reinterpret_cast<uint64_t *>(s_ax) : s_ct_own;
+  uint64_t *const s_rt = kF32 ? reinterpret_cast<uint64_t *>(s_ay) : s_rt_own;
   const int lane = threadIdx.x;
-  if (kTable && lane < 2 * kDiagsPerLane) s_rt[lane] = 0;
+  if (kTable && lane < SW::kRows) s_rt[lane] = 0;
   if (SW::kColTable)
     for (int q = lane; q < kDiagsPerLane * SW::kColStride; q += kWave) s_ct[q] = 0;
   const unsigned int nitems = (unsigned int)a.nitems;
@@ -1068,18 +1141,12 @@ void sweep_kernel(SweepArgs a) {
     if (lane == 0) b = atomicAdd(a.counter, 1u);
     b = (unsigned int)__builtin_amdgcn_readfirstlane((int)b);
     if (b >= nitems) break;
-    // FP64: the item is read where each field is needed (holding it in
-    // registers across the first segment costs the FP64 kernel spills);
-    // FP32: read whole up front (measured faster there)
-    const SweepItem *it = a.items + b;
-    SweepItem iv{};
-    if (sizeof(DT) == 4) iv = *it;
-    const int32_t band = sizeof(DT) == 4 ? iv.band : it->band;
+    ItemRows g;
+    if (!plan_item(a, b, g)) continue;
     // batched joins: the item's series, seen through offset pointers
     SweepArgs la = a;
     if (a.nbatch > 1) {
-      const int64_t o =
-          (int64_t)(sizeof(DT) == 4 ? iv.series : it->series) * a.batch_stride;
+      const int64_t o = g.series * a.batch_stride;
       la.dfX = static_cast<const DT *>(a.dfX) + o;
       la.dgX = static_cast<const DT *>(a.dgX) + o;
       la.inX = static_cast<const DT *>(a.inX) + o;
@@ -1093,24 +1160,19 @@ void sweep_kernel(SweepArgs a) {
       la.keyX = a.keyX + o;
       la.keyY = a.keyY + o;
     }
-    const int64_t D0 = la.d_base + (int64_t)band * kBand;
+    const int64_t D0 = la.d_base + g.band * kBand;
     Sweep<DT, KIND, COLS, ROWS, FRESH> sw(la, lane, D0);
     sw.s_kq = s_kq;
     if (kTable) sw.s_rt = s_rt;
     if (SW::kColTable) sw.s_ct = s_ct;
+    const int64_t dlast = D0 + kBand < la.d_hi ? D0 + kBand - 1 : la.d_hi - 1;
+    // all of the band's diagonals are valid on rows below rfull
+    const int64_t rfull = la.nY < la.nX - dlast ? la.nY : la.nX - dlast;
     for (int seg = 0; seg < 2; ++seg) {
-      const int64_t rs = sizeof(DT) == 4 ? (seg == 0 ? iv.rs : iv.rs2)
-                                         : (seg == 0 ? it->rs : it->rs2);
-      const int64_t re = sizeof(DT) == 4 ? (seg == 0 ? iv.re : iv.re2)
-                                         : (seg == 0 ? it->re : it->re2);
+      const int64_t rs = seg == 0 ? g.rs : g.rs2;
+      const int64_t re = seg == 0 ? g.re : g.re2;
       if (rs >= re) continue;
-      const int64_t dlast = D0 + kBand < la.d_hi ? D0 + kBand - 1 : la.d_hi - 1;
-      // all of the band's diagonals are valid on rows below rfull
-      const int64_t rfull = la.nY < la.nX - dlast ? la.nY : la.nX - dlast;
-      if (sizeof(DT) == 8)
-        sw.seed_from(la.seeds + (seg == 0 ? it->seed : it->seed2) + (int64_t)lane * K);
-      else
-        sw.anchor(rs, s_ax, s_ay);
+      sw.anchor(rs, s_ax, s_ay);
       sw.segment(rs, re, rfull, s_row, s_col, s_ax, s_ay);
     }
   }
