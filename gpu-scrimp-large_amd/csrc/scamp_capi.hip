@@ -29,6 +29,14 @@ using namespace scamp_amd;
 // the prep kernels' MatchRec mirrors this layout
 static_assert(sizeof(scamp_hip_match) == 12, "scamp_hip_match is 12 bytes");
 
+// Working memory of scamp_hip_join_device_noalloc: one copy per GPU, made
+// when the library's code object loads on that GPU and held until the
+// process ends.  In host code this name is not a device address:
+// scamp_hip_ctx_create looks up each device's copy with hipGetSymbolAddress,
+// and the join uses that pointer.
+constexpr size_t kScratchBytes = SCAMP_HIP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 // Calls on one context run one after another (each holds `mu`); separate
 // contexts, on the same device or not, run independently.
 struct scamp_hip_ctx {
@@ -40,6 +48,13 @@ struct scamp_hip_ctx {
   // value failed the mixed form's audit (then the join ran again in FP64)
   int last_precision = 0;
   uint64_t last_drift = 0;
+  // a join's timing events (start, statistics done, end, sweep start, sweep
+  // end), made once by scamp_hip_ctx_create
+  hipEvent_t ev[5] = {};
+  // this device's copy of the no-allocation join's scratch array (null when
+  // its address could not be obtained; scratch_status holds why)
+  void *scratch = nullptr;
+  hipError_t scratch_status = hipSuccess;
   std::mutex mu;
   // Device blocks kept from earlier calls (bytes, pointer), reused by later
   // ones: a short join would otherwise spend a noticeable part of its time
@@ -54,6 +69,18 @@ struct scamp_hip_ctx {
 namespace {
 
 thread_local std::string g_last_error;
+
+// Each device's scratch address, looked up once (with that device current)
+// by the first context made on it.  Joins through the scratch array on one
+// device take turns under that device's lock: they share its memory.
+constexpr int kMaxScratchDevices = 64;
+struct ScratchSlot {
+  std::once_flag once;
+  void *ptr = nullptr;
+  hipError_t status = hipErrorNotInitialized;
+  std::mutex mu;
+};
+ScratchSlot g_scratch[kMaxScratchDevices];
 
 // The mixed form's audit: an entry whose FP32 sweep correlation is further
 // than this from the FP64 correlation of the same pair (checked by the
@@ -96,16 +123,12 @@ struct HipFailure {
     if (e_ != hipSuccess) throw HipFailure{e_, #call};     \
   } while (0)
 
-// A HIP event owned by one scope (destroyed on every exit path).
-struct Event {
-  hipEvent_t e = nullptr;
-  Event() { HIP_CHECK(hipEventCreate(&e)); }
-  ~Event() {
-    if (e) (void)hipEventDestroy(e);
-  }
-  Event(const Event &) = delete;
-  Event &operator=(const Event &) = delete;
-};
+// Every working buffer is a whole number of 256-byte granules: blocks are
+// reused across nearby sizes, and carved blocks stay 256-byte aligned.
+template <typename T>
+size_t granules(int64_t count) {
+  return (sizeof(T) * (size_t)std::max<int64_t>(count, 1) + 255) / 256 * 256;
+}
 
 // Device allocations owned by one call (made with the context locked);
 // returned to the context's cache, or freed, on scope exit.  Work that still
@@ -119,9 +142,7 @@ struct DevArena {
   DevArena &operator=(const DevArena &) = delete;
   template <typename T>
   T *alloc(int64_t count) {
-    // 256-byte granules, so that blocks are reused across nearby sizes
-    const size_t bytes =
-        (sizeof(T) * (size_t)std::max<int64_t>(count, 1) + 255) / 256 * 256;
+    const size_t bytes = granules<T>(count);
     void *p = nullptr;
     // the smallest cached block that fits and wastes at most half of it
     size_t best = ctx->cache.size();
@@ -156,6 +177,36 @@ struct DevArena {
   }
 };
 
+// The no-allocation join's buffers: consecutive blocks of one device's
+// scratch array, handed out in the order the join asks for them.
+struct ScratchArena {
+  char *base;
+  size_t cap, used = 0;
+  ScratchArena(void *b, size_t c) : base(static_cast<char *>(b)), cap(c) {}
+  template <typename T>
+  T *alloc(int64_t count) {
+    const size_t bytes = granules<T>(count);
+    // the entry point checks the join's size first (SizeArena); this is
+    // the last guard against writing past the array
+    if (bytes > cap - used)
+      throw HipFailure{hipErrorOutOfMemory, "scratch array exhausted"};
+    T *p = reinterpret_cast<T *>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+// Counts the bytes a join's buffers take, through the same allocation
+// sequence (alloc_join) that hands them out.
+struct SizeArena {
+  size_t bytes = 0;
+  template <typename T>
+  T *alloc(int64_t count) {
+    bytes += granules<T>(count);
+    return nullptr;
+  }
+};
+
 // Statistics of one series, in the padded device layout the sweep reads.
 struct SeriesStats {
   int64_t n = 0, ns = 0;
@@ -164,44 +215,65 @@ struct SeriesStats {
   float *f_inorm = nullptr, *f_df = nullptr, *f_dg = nullptr;
 };
 
-SeriesStats make_stats(DevArena &ar, const double *d_T, int64_t n, int64_t m,
-                       bool fp32, hipStream_t s) {
+template <class Arena>
+SeriesStats alloc_stats(Arena &ar, int64_t n, int64_t m, bool fp32) {
   SeriesStats st;
   st.n = n;
   st.ns = n - m + 1;
-  const int64_t clean_len = n + kColumnPad;
   const int64_t stat_len = st.ns + kColumnPad;
-  st.clean = ar.alloc<double>(clean_len);
-  st.mean = ar.alloc<double>(stat_len);
-  st.inorm = ar.alloc<double>(stat_len);
-  st.df = ar.alloc<double>(stat_len);
-  st.dg = ar.alloc<double>(stat_len);
-  HIP_CHECK(launch_stats(d_T, n, m, st.clean, clean_len, st.mean, st.inorm,
-                         st.df, st.dg, stat_len, s));
+  st.clean = ar.template alloc<double>(n + kColumnPad);
+  st.mean = ar.template alloc<double>(stat_len);
+  st.inorm = ar.template alloc<double>(stat_len);
+  st.df = ar.template alloc<double>(stat_len);
+  st.dg = ar.template alloc<double>(stat_len);
   if (fp32) {
-    st.f_inorm = ar.alloc<float>(stat_len);
-    st.f_df = ar.alloc<float>(stat_len);
-    st.f_dg = ar.alloc<float>(stat_len);
-    HIP_CHECK(launch_to_f32(st.inorm, st.f_inorm, stat_len, s));
-    HIP_CHECK(launch_to_f32(st.df, st.f_df, stat_len, s));
-    HIP_CHECK(launch_to_f32(st.dg, st.f_dg, stat_len, s));
+    st.f_inorm = ar.template alloc<float>(stat_len);
+    st.f_df = ar.template alloc<float>(stat_len);
+    st.f_dg = ar.template alloc<float>(stat_len);
   }
   return st;
 }
 
+void compute_stats(const SeriesStats &st, const double *d_T, int64_t m,
+                   hipStream_t s) {
+  const int64_t stat_len = st.ns + kColumnPad;
+  HIP_CHECK(launch_stats(d_T, st.n, m, st.clean, st.n + kColumnPad, st.mean,
+                         st.inorm, st.df, st.dg, stat_len, s));
+  if (st.f_inorm) {
+    HIP_CHECK(launch_to_f32(st.inorm, st.f_inorm, stat_len, s));
+    HIP_CHECK(launch_to_f32(st.df, st.f_df, stat_len, s));
+    HIP_CHECK(launch_to_f32(st.dg, st.f_dg, stat_len, s));
+  }
+}
+
+SeriesStats make_stats(DevArena &ar, const double *d_T, int64_t n, int64_t m,
+                       bool fp32, hipStream_t s) {
+  SeriesStats st = alloc_stats(ar, n, m, fp32);
+  compute_stats(st, d_T, m, s);
+  return st;
+}
+
 // [lo, hi) minus the open interval (xlo, xhi): at most two ranges.
-std::vector<std::pair<int64_t, int64_t>> cut(int64_t lo, int64_t hi,
-                                             int64_t xlo, int64_t xhi) {
-  std::vector<std::pair<int64_t, int64_t>> out;
+struct Ranges {
+  int count = 0;
+  int64_t lo[2] = {}, hi[2] = {};
+  void add(int64_t l, int64_t h) {
+    lo[count] = l;
+    hi[count] = h;
+    ++count;
+  }
+};
+Ranges cut(int64_t lo, int64_t hi, int64_t xlo, int64_t xhi) {
+  Ranges out;
   if (lo >= hi) return out;
   // excluded integers: xlo < d < xhi
   int64_t ex0 = xlo + 1, ex1 = xhi;  // [ex0, ex1)
   if (ex0 >= ex1 || ex1 <= lo || ex0 >= hi) {
-    out.emplace_back(lo, hi);
+    out.add(lo, hi);
     return out;
   }
-  if (lo < ex0) out.emplace_back(lo, ex0);
-  if (ex1 < hi) out.emplace_back(ex1, hi);
+  if (lo < ex0) out.add(lo, ex0);
+  if (ex1 < hi) out.add(ex1, hi);
   return out;
 }
 
@@ -226,9 +298,14 @@ struct Pass {
 // Row chunks up to this length run the 1NN sweep's short-chunk variant.
 constexpr int64_t kFreshMaxRows = 16384;
 
-// Runs one pass (one diagonal range of one orientation).  sweep_ms: adds
-// the sweep's HIP-event time (null: no events, no wait for the sweep).
-void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
+// Passes of one join: a self-join has one; an AB-join up to two per
+// triangle (an aligned join's exclusion band can split a diagonal range).
+constexpr int kMaxPasses = 4;
+
+// Runs one pass (one diagonal range of one orientation); `counter` is its
+// work counter (one word, cleared here).  sweep_ms: adds the sweep's
+// HIP-event time (null: no events, no wait for the sweep).
+void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
               bool fp32, SweepKind kind, uint32_t *mat, int64_t mat_w,
               int64_t rpc, int64_t cpc, double thr, int64_t anchor, int knn_k,
               float *sweep_ms, int64_t nbatch = 1, int64_t stride = 0,
@@ -297,7 +374,6 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
     throw HipFailure{hipErrorInvalidValue, "too many sweep work items"};
 
   // the work counter, cleared by a fill kernel (no memset node)
-  uint64_t *counter = ar.alloc<uint64_t>(1);
   HIP_CHECK(launch_fill_u64(counter, 1, 0, s));
 
   SweepArgs a{};
@@ -329,13 +405,12 @@ void run_pass(scamp_hip_ctx *ctx, DevArena &ar, const Pass &p, int64_t m,
     HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, fresh));
     return;
   }
-  Event e0, e1;
-  HIP_CHECK(hipEventRecord(e0.e, s));
+  HIP_CHECK(hipEventRecord(ctx->ev[3], s));
   HIP_CHECK(launch_sweep(a, fp32, kind, p.cols, p.rows, grid, s, fresh));
-  HIP_CHECK(hipEventRecord(e1.e, s));
-  HIP_CHECK(hipEventSynchronize(e1.e));
+  HIP_CHECK(hipEventRecord(ctx->ev[4], s));
+  HIP_CHECK(hipEventSynchronize(ctx->ev[4]));
   float ms = 0;
-  HIP_CHECK(hipEventElapsedTime(&ms, e0.e, e1.e));
+  HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
   *sweep_ms += ms;
 }
 
@@ -362,67 +437,138 @@ const char *validate(const scamp_hip_params *p, int64_t na, int64_t nb) {
   return nullptr;
 }
 
-int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
-                const double *d_b, int64_t nb, const scamp_hip_params *p,
-                void *d_prof_a, void *d_prof_b) {
-  const int64_t m = p->window;
-  const bool self = !p->has_b;
-  if (self) {
-    d_b = d_a;
-    nb = na;
-  }
+// What one join's buffers depend on.
+struct JoinShape {
+  int64_t na = 0, nb = 0, m = 0;
+  bool self = true, fp32 = false, sep_rows = false, refine = false;
+  int pt = 0, knn_k = 0;
+  int64_t H = 0, W = 0;
+};
+
+// Checks the parameters a join adds to validate() and fills `js` (nb = na
+// for a self-join).  Returns SCAMP_HIP_OK, or fail()'s code and message.
+int join_shape(const scamp_hip_params *p, int64_t na, int64_t nb, JoinShape *js) {
   const int pt = p->profile_type;
   if (pt != SCAMP_HIP_1NN_INDEX && pt != SCAMP_HIP_1NN &&
       pt != SCAMP_HIP_SUM_THRESH && pt != SCAMP_HIP_MATRIX_SUMMARY &&
       pt != SCAMP_HIP_ALL_NEIGHBORS)
     return fail(SCAMP_HIP_UNIMPLEMENTED,
                 "profile type not supported by this engine");
-  int prec = p->precision;
+  const int prec = p->precision;
   if (prec != SCAMP_HIP_SINGLE && prec != SCAMP_HIP_MIXED &&
       prec != SCAMP_HIP_DOUBLE && prec != SCAMP_HIP_ULTRA)
     return fail(SCAMP_HIP_INVALID_ARGS, "invalid precision type");
-  const bool fp32 = prec == SCAMP_HIP_MIXED || prec == SCAMP_HIP_SINGLE;
   if (pt == SCAMP_HIP_MATRIX_SUMMARY &&
       (p->matrix_height <= 0 || p->matrix_width <= 0))
     return fail(SCAMP_HIP_INVALID_ARGS, "matrix height and width must be > 0");
+  js->m = p->window;
+  js->self = !p->has_b;
+  js->na = na;
+  js->nb = js->self ? na : nb;
+  js->fp32 = prec == SCAMP_HIP_MIXED || prec == SCAMP_HIP_SINGLE;
+  js->pt = pt;
+  js->sep_rows = p->computing_rows && p->keep_rows_separate;
+  // The mixed form's 1NN_INDEX profiles are refined in FP64 after the sweep
+  // (refine_kernel.hip); entries still equal to the prior are left alone.
+  js->refine = js->fp32 && pt == SCAMP_HIP_1NN_INDEX;
+  js->knn_k = (int)p->max_matches_per_column;
+  js->H = p->matrix_height;
+  js->W = p->matrix_width;
+  return SCAMP_HIP_OK;
+}
+
+// The working buffers of one join, taken from `ar` in one fixed order
+// before any kernel writes the caller's memory.
+struct JoinBuffers {
+  SeriesStats A, B;
+  // working profiles (padded); ALL_NEIGHBORS lists; the matrix summary
+  uint64_t *keyA = nullptr, *keyB = nullptr, *knnA = nullptr, *knnB = nullptr;
+  uint32_t *mat = nullptr;
+  // the mixed form's priors and audit counter; one work counter per pass
+  uint64_t *priorA = nullptr, *priorB = nullptr, *drift = nullptr;
+  uint64_t *counters = nullptr;
+};
+
+template <class Arena>
+JoinBuffers alloc_join(Arena &ar, const JoinShape &js) {
+  JoinBuffers b;
+  const int64_t nsA = js.na - js.m + 1, nsB = js.nb - js.m + 1;
+  b.A = alloc_stats(ar, js.na, js.m, js.fp32);
+  b.B = js.self ? b.A : alloc_stats(ar, js.nb, js.m, js.fp32);
+  if (js.pt == SCAMP_HIP_ALL_NEIGHBORS) {
+    b.keyA = ar.template alloc<uint64_t>(nsA + kColumnPad);
+    b.knnA = ar.template alloc<uint64_t>(nsA * js.knn_k);
+    if (js.sep_rows) {
+      b.keyB = ar.template alloc<uint64_t>(nsB + kColumnPad);
+      b.knnB = ar.template alloc<uint64_t>(nsB * js.knn_k);
+    }
+  } else if (js.pt == SCAMP_HIP_1NN_INDEX || js.pt == SCAMP_HIP_1NN) {
+    b.keyA = ar.template alloc<uint64_t>(nsA + kColumnPad);
+    if (js.sep_rows) b.keyB = ar.template alloc<uint64_t>(nsB + kColumnPad);
+  } else if (js.pt == SCAMP_HIP_MATRIX_SUMMARY) {
+    b.mat = ar.template alloc<uint32_t>(js.H * js.W);
+  }
+  if (js.refine) {
+    b.drift = ar.template alloc<uint64_t>(1);
+    b.priorA = ar.template alloc<uint64_t>(nsA);
+    if (js.sep_rows) b.priorB = ar.template alloc<uint64_t>(nsB);
+  }
+  b.counters = ar.template alloc<uint64_t>(kMaxPasses);
+  return b;
+}
+
+// Bytes of device memory a join's working buffers take.
+size_t join_bytes(const JoinShape &js) {
+  SizeArena sz;
+  (void)alloc_join(sz, js);
+  return sz.bytes;
+}
+
+template <class Arena>
+int join_device(scamp_hip_ctx *ctx, Arena &ar, const double *d_a, int64_t na,
+                const double *d_b, int64_t nb, const scamp_hip_params *p,
+                void *d_prof_a, void *d_prof_b) {
+  JoinShape js;
+  if (int rc = join_shape(p, na, nb, &js)) return rc;
+  const int64_t m = js.m;
+  const bool self = js.self;
+  if (self) d_b = d_a;
+  const int pt = js.pt;
+  const int prec = p->precision;
+  const bool fp32 = js.fp32;
   const int64_t anchor = anchor_rows(p);
 
   hipStream_t s = ctx->stream;
-  DevArena ar(ctx);
-  Event ev_start, ev_prep, ev_end;
-  HIP_CHECK(hipEventRecord(ev_start.e, s));
+  hipEvent_t ev_start = ctx->ev[0], ev_prep = ctx->ev[1], ev_end = ctx->ev[2];
+  HIP_CHECK(hipEventRecord(ev_start, s));
 
-  SeriesStats A = make_stats(ar, d_a, na, m, fp32, s);
-  SeriesStats B = self ? A : make_stats(ar, d_b, nb, m, fp32, s);
+  JoinBuffers bf = alloc_join(ar, js);
+  const SeriesStats &A = bf.A, &B = bf.B;
+  compute_stats(A, d_a, m, s);
+  if (!self) compute_stats(B, d_b, m, s);
   const int64_t nsA = A.ns, nsB = B.ns;
   const bool rows_out = p->computing_rows && (self || p->keep_rows_separate);
   const bool cols_out = p->computing_columns != 0;
-  const bool sep_rows = p->computing_rows && p->keep_rows_separate;
+  const bool sep_rows = js.sep_rows;
 
   // Working profiles (padded) initialised from the caller's prior.
-  uint64_t *keyA = nullptr, *keyB = nullptr;
+  uint64_t *keyA = bf.keyA, *keyB = bf.keyB, *knnA = bf.knnA, *knnB = bf.knnB;
   double *sumA = nullptr, *sumB = nullptr;
-  uint64_t *knnA = nullptr, *knnB = nullptr;
-  uint32_t *mat = nullptr;
-  const int64_t H = p->matrix_height, W = p->matrix_width;
-  const int knn_k = (int)p->max_matches_per_column;
+  uint32_t *mat = bf.mat;
+  const int64_t H = js.H, W = js.W;
+  const int knn_k = js.knn_k;
   // ALL_NEIGHBORS: a match enters a list when its key beats `knn_init`,
   // i.e. when its FP32 correlation is above the threshold
   const uint64_t knn_init = make_key((float)p->distance_threshold, 0u);
   if (pt == SCAMP_HIP_ALL_NEIGHBORS) {
-    keyA = ar.alloc<uint64_t>(nsA + kColumnPad);
     HIP_CHECK(launch_fill_u64(keyA + nsA, kColumnPad, knn_init, s));
-    knnA = ar.alloc<uint64_t>(nsA * knn_k);
     HIP_CHECK(launch_matches_to_knn(d_prof_a, nsA, knn_k, knn_init, knnA, keyA, s));
     if (sep_rows) {
-      keyB = ar.alloc<uint64_t>(nsB + kColumnPad);
       HIP_CHECK(launch_fill_u64(keyB + nsB, kColumnPad, knn_init, s));
-      knnB = ar.alloc<uint64_t>(nsB * knn_k);
       HIP_CHECK(launch_matches_to_knn(d_prof_b, nsB, knn_k, knn_init, knnB, keyB, s));
     }
   } else if (pt == SCAMP_HIP_1NN_INDEX || pt == SCAMP_HIP_1NN) {
     const uint64_t init = make_key(-2.0f, 0xFFFFFFFFu);
-    keyA = ar.alloc<uint64_t>(nsA + kColumnPad);
     HIP_CHECK(launch_fill_u64(keyA + nsA, kColumnPad, init, s));
     if (pt == SCAMP_HIP_1NN_INDEX) {
       HIP_CHECK(hipMemcpyAsync(keyA, d_prof_a, sizeof(uint64_t) * nsA,
@@ -432,7 +578,6 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
       HIP_CHECK(launch_f32_to_keys((const float *)d_prof_a, keyA, nsA, s));
     }
     if (sep_rows) {
-      keyB = ar.alloc<uint64_t>(nsB + kColumnPad);
       HIP_CHECK(launch_fill_u64(keyB + nsB, kColumnPad, init, s));
       if (pt == SCAMP_HIP_1NN_INDEX) {
         HIP_CHECK(hipMemcpyAsync(keyB, d_prof_b, sizeof(uint64_t) * nsB,
@@ -447,28 +592,21 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     sumA = static_cast<double *>(d_prof_a);
     if (sep_rows) sumB = static_cast<double *>(d_prof_b);
   } else {
-    mat = ar.alloc<uint32_t>(H * W);
     HIP_CHECK(hipMemcpyAsync(mat, d_prof_a, sizeof(float) * H * W,
                              hipMemcpyDeviceToDevice, s));
     HIP_CHECK(launch_f32_to_enc(reinterpret_cast<float *>(mat), H * W, s));
   }
-  // The mixed form's 1NN_INDEX profiles are refined in FP64 after the sweep
-  // (refine_kernel.hip); entries still equal to the prior are left alone.
-  const bool refine = fp32 && pt == SCAMP_HIP_1NN_INDEX;
-  uint64_t *priorA = nullptr, *priorB = nullptr, *drift = nullptr;
+  const bool refine = js.refine;
+  uint64_t *priorA = bf.priorA, *priorB = bf.priorB, *drift = bf.drift;
   if (refine) {
-    drift = ar.alloc<uint64_t>(1);
     HIP_CHECK(launch_fill_u64(drift, 1, 0, s));
-    priorA = ar.alloc<uint64_t>(nsA);
     HIP_CHECK(hipMemcpyAsync(priorA, keyA, sizeof(uint64_t) * nsA,
                              hipMemcpyDeviceToDevice, s));
-    if (sep_rows) {
-      priorB = ar.alloc<uint64_t>(nsB);
+    if (sep_rows)
       HIP_CHECK(hipMemcpyAsync(priorB, keyB, sizeof(uint64_t) * nsB,
                                hipMemcpyDeviceToDevice, s));
-    }
   }
-  HIP_CHECK(hipEventRecord(ev_prep.e, s));
+  HIP_CHECK(hipEventRecord(ev_prep, s));
 
   // Passes.  Self-join: diagonals [m/4, nsA) of the upper triangle, each pair
   // updating its row and its column in the same profile (src/tile.cpp:12-38
@@ -496,14 +634,16 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     HIP_CHECK(launch_warm_profile(A.clean, A.mean, A.inorm, nsA, m, e, 1, 0,
                                   cols_out ? keyA : nullptr,
                                   rows_out ? (sep_rows ? keyB : keyA) : nullptr, s));
-  std::vector<Pass> passes;
+  Pass passes[kMaxPasses];
+  int npass = 0;
   if (self) {
-    for (auto &r : cut(e, nsA, 0, 0)) {
-      Pass ps{&A, &A, r.first, r.second, cols_out, rows_out, keyA,
+    const Ranges r = cut(e, nsA, 0, 0);
+    for (int i = 0; i < r.count; ++i) {
+      Pass ps{&A, &A, r.lo[i], r.hi[i], cols_out, rows_out, keyA,
               sep_rows ? keyB : keyA, sumA, sep_rows ? sumB : sumA,
               knnA, sep_rows ? knnB : knnA};
       if (pt == SCAMP_HIP_MATRIX_SUMMARY) ps.cols = ps.rows = true;
-      passes.push_back(ps);
+      passes[npass++] = ps;
     }
   } else {
     int64_t delta = 0;
@@ -511,31 +651,31 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
       delta = p->distributed_start_col - p->distributed_start_row;
     const bool al = p->is_aligned != 0 && e > 0;
     // upper: d = j - i; excluded when |d + delta| < e
-    for (auto &r : al ? cut(0, nsA, -e - delta, e - delta)
-                      : cut(0, nsA, 0, 0)) {
-      Pass ps{&A, &B, r.first, r.second, cols_out, sep_rows, keyA, keyB,
+    const Ranges up = al ? cut(0, nsA, -e - delta, e - delta) : cut(0, nsA, 0, 0);
+    for (int i = 0; i < up.count; ++i) {
+      Pass ps{&A, &B, up.lo[i], up.hi[i], cols_out, sep_rows, keyA, keyB,
               sumA, sumB, knnA, knnB};
       if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
         ps.cols = true;
         ps.rows = false;
       }
-      passes.push_back(ps);
+      passes[npass++] = ps;
     }
     // lower (transposed): d = i - j >= 1; excluded when |d - delta| < e
-    for (auto &r : al ? cut(1, nsB, delta - e, delta + e)
-                      : cut(1, nsB, 0, 0)) {
-      Pass ps{&B, &A, r.first, r.second, sep_rows, cols_out, keyB, keyA,
+    const Ranges lo = al ? cut(1, nsB, delta - e, delta + e) : cut(1, nsB, 0, 0);
+    for (int i = 0; i < lo.count; ++i) {
+      Pass ps{&B, &A, lo.lo[i], lo.hi[i], sep_rows, cols_out, keyB, keyA,
               sumB, sumA, knnB, knnA};
       if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
         ps.cols = false;
         ps.rows = true;
       }
-      passes.push_back(ps);
+      passes[npass++] = ps;
     }
   }
   float sweep_ms = 0;
-  for (const Pass &ps : passes)
-    run_pass(ctx, ar, ps, m, fp32, kind, mat, W, rpc, cpc,
+  for (int i = 0; i < npass; ++i)
+    run_pass(ctx, bf.counters + i, passes[i], m, fp32, kind, mat, W, rpc, cpc,
              p->distance_threshold, anchor, knn_k, &sweep_ms);
 
   if (refine) {
@@ -571,9 +711,9 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
       // the caller's profiles are untouched so far (the result lives in the
       // working keys): the join runs again in FP64
       float spent = 0;
-      HIP_CHECK(hipEventRecord(ev_end.e, s));
-      HIP_CHECK(hipEventSynchronize(ev_end.e));
-      HIP_CHECK(hipEventElapsedTime(&spent, ev_start.e, ev_end.e));
+      HIP_CHECK(hipEventRecord(ev_end, s));
+      HIP_CHECK(hipEventSynchronize(ev_end));
+      HIP_CHECK(hipEventElapsedTime(&spent, ev_start, ev_end));
       ctx->total_ms = spent;
       ctx->sweep_ms = sweep_ms;
       if (!p->silent_mode)
@@ -606,10 +746,10 @@ int join_device(scamp_hip_ctx *ctx, const double *d_a, int64_t na,
     HIP_CHECK(hipMemcpyAsync(d_prof_a, mat, sizeof(float) * H * W,
                              hipMemcpyDeviceToDevice, s));
   }
-  HIP_CHECK(hipEventRecord(ev_end.e, s));
-  HIP_CHECK(hipEventSynchronize(ev_end.e));
+  HIP_CHECK(hipEventRecord(ev_end, s));
+  HIP_CHECK(hipEventSynchronize(ev_end));
   float total = 0;
-  HIP_CHECK(hipEventElapsedTime(&total, ev_start.e, ev_end.e));
+  HIP_CHECK(hipEventElapsedTime(&total, ev_start, ev_end));
   ctx->total_ms = total;
   ctx->sweep_ms = sweep_ms;
   ctx->prep_ms = total - sweep_ms;
@@ -638,8 +778,8 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   const int64_t seg = (n + kColumnPad + 63) / 64 * 64;
   hipStream_t s = ctx->stream;
   DevArena ar(ctx);
-  Event ev_start, ev_end;
-  HIP_CHECK(hipEventRecord(ev_start.e, s));
+  hipEvent_t ev_start = ctx->ev[0], ev_end = ctx->ev[2];
+  HIP_CHECK(hipEventRecord(ev_start, s));
 
   SeriesStats A;
   A.n = n;
@@ -684,12 +824,14 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   HIP_CHECK(launch_warm_profile(A.clean, A.mean, A.inorm, ns, m, m / 4, batch,
                                 seg, cols ? key : nullptr, rows ? key : nullptr,
                                 s));
+  uint64_t *counters = ar.alloc<uint64_t>(kMaxPasses);
   float sweep_ms = 0;
-  for (auto &r : cut(m / 4, ns, 0, 0)) {
-    Pass ps{&A, &A, r.first, r.second, cols, rows, key, key,
+  const Ranges r = cut(m / 4, ns, 0, 0);
+  for (int i = 0; i < r.count; ++i) {
+    Pass ps{&A, &A, r.lo[i], r.hi[i], cols, rows, key, key,
             nullptr, nullptr, nullptr, nullptr};
-    run_pass(ctx, ar, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor, 0,
-             &sweep_ms, batch, seg, ns);
+    run_pass(ctx, counters + i, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor,
+             0, &sweep_ms, batch, seg, ns);
   }
   if (refine) {
     HIP_CHECK(launch_refine(key, prior, ns, A.clean, A.mean, A.inorm, A.df, A.dg,
@@ -703,9 +845,9 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
     ctx->last_drift = h_drift;
     if (h_drift > 0) {
       float spent = 0;
-      HIP_CHECK(hipEventRecord(ev_end.e, s));
-      HIP_CHECK(hipEventSynchronize(ev_end.e));
-      HIP_CHECK(hipEventElapsedTime(&spent, ev_start.e, ev_end.e));
+      HIP_CHECK(hipEventRecord(ev_end, s));
+      HIP_CHECK(hipEventSynchronize(ev_end));
+      HIP_CHECK(hipEventElapsedTime(&spent, ev_start, ev_end));
       ctx->total_ms = spent;
       ctx->sweep_ms = sweep_ms;
       if (!p->silent_mode)
@@ -717,10 +859,10 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   }
 
   HIP_CHECK(launch_profile_out(key, seg, d_prof, mp, ns, batch, s));
-  HIP_CHECK(hipEventRecord(ev_end.e, s));
-  HIP_CHECK(hipEventSynchronize(ev_end.e));
+  HIP_CHECK(hipEventRecord(ev_end, s));
+  HIP_CHECK(hipEventSynchronize(ev_end));
   float total = 0;
-  HIP_CHECK(hipEventElapsedTime(&total, ev_start.e, ev_end.e));
+  HIP_CHECK(hipEventElapsedTime(&total, ev_start, ev_end));
   ctx->total_ms = total;
   ctx->sweep_ms = sweep_ms;
   ctx->prep_ms = total - sweep_ms;
@@ -779,19 +921,34 @@ int scamp_hip_ctx_create(int device, scamp_hip_ctx **out) {
     return fail(SCAMP_HIP_NO_DEVICE, "no HIP device is visible");
   if (device < 0 || device >= n)
     return fail(SCAMP_HIP_INVALID_ARGS, "device id out of range");
+  auto *ctx = new scamp_hip_ctx();
+  ctx->device = device;
   try {
-    auto *ctx = new scamp_hip_ctx();
-    ctx->device = device;
     HIP_CHECK(hipSetDevice(device));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     ctx->num_cus = prop.multiProcessorCount;
     HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    *out = ctx;
+    for (hipEvent_t &e : ctx->ev) HIP_CHECK(hipEventCreate(&e));
   } catch (const HipFailure &f) {
+    scamp_hip_ctx_destroy(ctx);
     return fail(SCAMP_HIP_DEVICE_ERROR,
                 std::string(f.what) + ": " + hipGetErrorString(f.err));
   }
+  // this device's scratch array (its own copy, at its own address)
+  if (device < kMaxScratchDevices) {
+    ScratchSlot &sl = g_scratch[device];
+    std::call_once(sl.once, [&sl] {
+      void *ptr = nullptr;
+      sl.status = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (sl.status == hipSuccess) sl.ptr = ptr;
+    });
+    ctx->scratch = sl.ptr;
+    ctx->scratch_status = sl.status;
+  } else {
+    ctx->scratch_status = hipErrorInvalidDevice;
+  }
+  *out = ctx;
   return SCAMP_HIP_OK;
 }
 
@@ -800,6 +957,8 @@ int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &b : ctx->cache) (void)hipFree(b.second);
+  for (hipEvent_t e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SCAMP_HIP_OK;
@@ -829,8 +988,57 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
     return with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
-      return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, q, d_profile_a,
-                         d_profile_b);
+      DevArena ar(ctx);
+      return join_device(ctx, ar, d_a, (int64_t)na, d_b, (int64_t)nb, q,
+                         d_profile_a, d_profile_b);
+    });
+  } catch (const HipFailure &f) {
+    (void)hipStreamSynchronize(ctx->stream);
+    int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
+                                            : SCAMP_HIP_DEVICE_ERROR;
+    return fail(code, std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
+}
+
+int scamp_hip_join_device_noalloc(scamp_hip_ctx *ctx, const double *d_a,
+                                  size_t na, const double *d_b, size_t nb,
+                                  const scamp_hip_params *p, void *d_profile_a,
+                                  void *d_profile_b) {
+  if (!ctx || !p || !d_a || !d_profile_a)
+    return fail(SCAMP_HIP_INVALID_ARGS, "null argument");
+  if (p->has_b && !d_b) return fail(SCAMP_HIP_INVALID_ARGS, "missing series b");
+  if (const char *msg = validate(p, (int64_t)na, (int64_t)nb))
+    return fail(SCAMP_HIP_INVALID_ARGS, msg);
+  if (p->computing_rows && p->keep_rows_separate && !d_profile_b &&
+      p->profile_type != SCAMP_HIP_MATRIX_SUMMARY)
+    return fail(SCAMP_HIP_INVALID_ARGS, "missing profile b");
+  JoinShape js;
+  if (int rc = join_shape(p, (int64_t)na, (int64_t)nb, &js)) return rc;
+  char msg[192];
+  if (!ctx->scratch) {
+    std::snprintf(msg, sizeof msg,
+                  "no-allocation join: no scratch array on device %d (%s)",
+                  ctx->device, hipGetErrorString(ctx->scratch_status));
+    return fail(SCAMP_HIP_DEVICE_ERROR, msg);
+  }
+  // the requested precision's buffers; an FP64 rerun after a failed audit
+  // needs fewer (no FP32 terms, no priors)
+  const size_t need = join_bytes(js);
+  if (need > kScratchBytes) {
+    std::snprintf(msg, sizeof msg,
+                  "no-allocation join: needs %zu bytes of working memory, the "
+                  "scratch array holds %zu",
+                  need, kScratchBytes);
+    return fail(SCAMP_HIP_INVALID_ARGS, msg);
+  }
+  try {
+    std::lock_guard<std::mutex> dev_lock(g_scratch[ctx->device].mu);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_CHECK(hipSetDevice(ctx->device));
+    return with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
+      ScratchArena ar(ctx->scratch, kScratchBytes);
+      return join_device(ctx, ar, d_a, (int64_t)na, d_b, (int64_t)nb, q,
+                         d_profile_a, d_profile_b);
     });
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
@@ -917,7 +1125,9 @@ int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
       HIP_CHECK(hipMemcpyAsync(d_pb, profile_b, eb * len_b, hipMemcpyHostToDevice, s));
     }
     int rc = with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
-      return join_device(ctx, d_a, (int64_t)na, d_b, (int64_t)nb, q, d_pa, d_pb);
+      DevArena work(ctx);
+      return join_device(ctx, work, d_a, (int64_t)na, d_b, (int64_t)nb, q, d_pa,
+                         d_pb);
     });
     if (rc != SCAMP_HIP_OK) return rc;
     HIP_CHECK(hipMemcpyAsync(profile_a, d_pa, eb * len_a, hipMemcpyDeviceToHost, s));

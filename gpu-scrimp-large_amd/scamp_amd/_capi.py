@@ -29,7 +29,9 @@ EXPORTS = ("scamp_hip_ctx_create", "scamp_hip_ctx_destroy", "scamp_hip_device_co
            "scamp_hip_last_error", "scamp_hip_join", "scamp_hip_join_device",
            "scamp_hip_stats", "scamp_hip_qt_seed", "scamp_hip_last_timing",
            "scamp_hip_pearson_backward", "scamp_hip_selfjoin_batch",
-           "scamp_hip_last_precision")
+           "scamp_hip_last_precision", "scamp_hip_join_device_noalloc")
+# SCAMP_HIP_NOALLOC_SCRATCH_BYTES: the no-allocation join's working memory per GPU
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 
 class Params(ctypes.Structure):
@@ -80,6 +82,8 @@ def load():
     lib.scamp_hip_last_error.argtypes = [ctypes.c_char_p, sz]
     lib.scamp_hip_join.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(Params), vp, vp]
     lib.scamp_hip_join_device.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(Params), vp, vp]
+    lib.scamp_hip_join_device_noalloc.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(Params),
+                                                  vp, vp]
     lib.scamp_hip_stats.argtypes = [vp, vp, sz, i64, vp, vp, vp, vp]
     lib.scamp_hip_qt_seed.argtypes = [vp, vp, sz, vp, sz, i64, i64, i64, i64, vp]
     lib.scamp_hip_last_timing.argtypes = [vp, dp, dp, dp]
@@ -266,6 +270,18 @@ def join_device(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=None, devic
     (e.g. torch tensors' data_ptr()).  Blocks until the profiles are written."""
     ctx = context(device)
     check(ctx._lib.scamp_hip_join_device(
+        ctx.handle, ctypes.c_void_p(ptr_a), na, ctypes.c_void_p(ptr_b) if ptr_b else None,
+        nb, ctypes.byref(params), ctypes.c_void_p(ptr_pa),
+        ctypes.c_void_p(ptr_pb) if ptr_pb else None))
+
+
+def join_device_noalloc(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=None,
+                        device=0):
+    """join_device without allocating (scamp_hip_join_device_noalloc): its
+    working buffers come from the device's fixed scratch array of
+    NOALLOC_SCRATCH_BYTES; larger joins raise ScampError (INVALID_ARGS)."""
+    ctx = context(device)
+    check(ctx._lib.scamp_hip_join_device_noalloc(
         ctx.handle, ctypes.c_void_p(ptr_a), na, ctypes.c_void_p(ptr_b) if ptr_b else None,
         nb, ctypes.byref(params), ctypes.c_void_p(ptr_pa),
         ctypes.c_void_p(ptr_pb) if ptr_pb else None))

@@ -198,6 +198,30 @@ int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
 int scamp_hip_last_precision(scamp_hip_ctx *ctx, int *precision,
                              uint64_t *drift_entries);
 
+/* Working memory of scamp_hip_join_device_noalloc: one array of this many
+ * bytes per GPU, held from the library's first context on that GPU until the
+ * process ends. */
+#define SCAMP_HIP_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* scamp_hip_join_device without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as scamp_hip_join_device; every working
+ * buffer (statistics, working profiles, counters) is carved from the
+ * device's scratch array instead of hipMalloc, and the call makes no
+ * hipMalloc, hipFree or event creation.  A join whose buffers exceed
+ * SCAMP_HIP_NOALLOC_SCRATCH_BYTES is refused with SCAMP_HIP_INVALID_ARGS
+ * (about 2.7 million subsequences for an FP64 1NN_INDEX self-join, 1.9
+ * million for MIXED).  All calls on one GPU share its array, which keeps the
+ * last call's contents (each call initialises what it reads): calls on one
+ * GPU take turns under a per-device lock inside the library, and any other
+ * code that overlapped them on the same array would race.  Blocks until
+ * done. */
+int scamp_hip_join_device_noalloc(scamp_hip_ctx *ctx, const double *d_a,
+                                  size_t na, const double *d_b, size_t nb,
+                                  const scamp_hip_params *p, void *d_profile_a,
+                                  void *d_profile_b);
+
 #ifdef __cplusplus
 }
 #endif

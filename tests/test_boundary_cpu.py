@@ -106,16 +106,18 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         pytest.skip("no C compiler")
     src = tmp_path / "sz.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "scamp_hip.h"\n'
-                   'int main(void) { printf("%zu %zu %zu %zu\\n", sizeof(scamp_hip_params), '
+                   'int main(void) { printf("%zu %zu %zu %zu %zu\\n", sizeof(scamp_hip_params), '
                    'offsetof(scamp_hip_params, anchor_interval), '
                    'offsetof(scamp_hip_params, max_matches_per_column), '
-                   'sizeof(scamp_hip_match)); return 0; }\n')
+                   'sizeof(scamp_hip_match), SCAMP_HIP_NOALLOC_SCRATCH_BYTES); '
+                   'return 0; }\n')
     exe = tmp_path / "sz"
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    size, off_anchor, off_k, match = (int(x) for x in out)
+    size, off_anchor, off_k, match, scratch = (int(x) for x in out)
     assert size == ctypes.sizeof(C.Params)
     assert off_anchor == C.Params.anchor_interval.offset
     assert off_k == C.Params.max_matches_per_column.offset
     assert match == C.MATCH_DTYPE.itemsize == 12
+    assert scratch == C.NOALLOC_SCRATCH_BYTES
