@@ -84,9 +84,12 @@ struct Terms<double> {
 // Integer image of an FP64 threshold t for the two-stage filter: for c >= 0,
 // c >= t implies hi32(c) >= enc_thr(t) (the high word of a non-negative
 // double orders like the double and truncates it downwards); a negative
-// threshold passes every correlation (kNegEnc decodes to -1.0625 with a
-// zero low word); a window that can never match gets kNanEnc, which no
-// correlation's high word reaches and which decodes to NaN.
+// threshold maps to kNegEnc, below every non-negative image, which decodes
+// to -1.0625 with a zero low word (the first stage then passes every
+// correlation) and which the exact column test treats as "take any
+// correlation" (col_passes: negative doubles' high words order the other
+// way round as integers); a window that can never match gets kNanEnc, which
+// no correlation's high word reaches and which decodes to NaN.
 constexpr int32_t kNegEnc = (int32_t)0xBFF10000u;
 constexpr int32_t kNanEnc = 0x7FFFFFFF;
 __device__ __forceinline__ int32_t enc_thr(double t) {
@@ -586,9 +589,12 @@ struct Sweep {
   // (FRESH: into the block's column table).
   __device__ __forceinline__ bool col_passes(DT c, const Slot<DT> &s) const {
     // the FP64 two-stage form tests the integer image (conservative; the
-    // NaN of an invalid diagonal passes it and is caught here)
+    // NaN of an invalid diagonal passes it and is caught here).  The high
+    // word of a negative double orders the other way round as an integer,
+    // so a column whose threshold is negative (kNegEnc) takes every
+    // correlation, negative ones included.
     if constexpr (kIntCol)
-      return c == c && __double2hiint(c) >= s.F;
+      return c == c && (__double2hiint(c) >= s.F || s.F == kNegEnc);
     else
       return c >= s.t;
   }
