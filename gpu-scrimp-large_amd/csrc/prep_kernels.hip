@@ -364,8 +364,10 @@ inline unsigned blocks_for(int64_t n, int bs) {
 // Covariances:
 //   sum_t x_t (T[j + t] - mu_i) - (mu_j - mu_i) sum_t x_t,  x_t = T[i + t] - mu_i
 // (both factors centred near the windows' level, so a large offset of the
-// series costs no digits); the correlation is formed in the sweep's order,
-// (cov * inorm_col) * inorm_row.  A block's windows and their partners are
+// series costs no digits); the correlation is formed as
+// (cov * inorm_col) * inorm_row, the short-chunk sweep's order (the
+// long-chunk two-stage kernels form (cov * inorm_row) * inorm_col; the two
+// differ by at most one FP64 rounding).  A block's windows and their partners are
 // staged through LDS, kWarmChunk samples of t at a time; each thread slides a
 // register window over the partners so that every staged sample feeds kWarm
 // FMAs.  The column sides of the block's pairs meet in an LDS table (one key

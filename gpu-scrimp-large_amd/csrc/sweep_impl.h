@@ -60,9 +60,10 @@ template <typename DT>
 struct Slot {
   DT df, dg, in;   // column update terms and inverse norm
   DT t;            // 1NN: threshold of the column's current best
-  // Two-stage 1NN filter (see Sweep::kTwoStage): mt = min(block row bound,
-  // column threshold), the one-compare first stage; the FP64 form keeps its
-  // column threshold as the integer image F (enc_thr) instead of t.
+  // Two-stage 1NN filter (see Sweep::kTwoStage): mt = the first-stage bound
+  // min(block row bound, column threshold) / in (row-scaled units); the FP64
+  // form keeps its column threshold as the integer image F (enc_thr)
+  // instead of t.
   DT mt;
   int32_t F;
   double acc;      // SUM: partial sum of the column, travelling with it
@@ -73,6 +74,7 @@ template <typename DT>
 struct Terms {
   DT df, dg, in;
   DT t;       // 1NN: threshold of the row's / column's current best
+  DT b;       // FP32 two-stage filter: the column's first-stage bound
 };
 template <>
 struct Terms<double> {
@@ -109,6 +111,26 @@ __device__ __forceinline__ float min_thr(float a, float b) {
 // either threshold it stands for).
 __device__ __forceinline__ double mt_of(int32_t tlo, int32_t f) {
   return __hiloint2double(tlo < f ? tlo : f, 0);
+}
+// The FP64 first stage in row-scaled units: a pair passes when
+// s = cov * in_row >= B, B = mt_of(tlo, f) / in_col less a relative 2^-30,
+// which (s * in_col >= threshold implies s >= B under FP64 rounding, with
+// the reciprocal good to far better than 2^-31 after one Newton step) keeps
+// the stage conservative.  A column that can never match has in_col = NaN,
+// so B = NaN and no pair passes.
+__device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, double in_col) {
+  double r = __builtin_amdgcn_rcp(in_col);
+  r = fma(r, fma(-in_col, r, 1.0), r);
+  const double b = mt_of(tlo, f) * r;
+  return fma(-fabs(b), 0x1p-30, b);
+}
+
+// The FP32 first stage in row-scaled units (as stage1_bound): B =
+// min(tlo, t) / in_col less a relative 2^-16, far above the FP32 rounding of
+// the product and of the hardware reciprocal.
+__device__ __forceinline__ float stage1_bound32(float tlo, float t, float in_col) {
+  const float b = min_thr(tlo, t) * __builtin_amdgcn_rcpf(in_col);
+  return fmaf(-fabsf(b), 0x1p-16f, b);
 }
 
 template <typename DT>
@@ -188,12 +210,16 @@ struct Sweep {
                   : K;
   static_assert(kRows <= kWave, "one staged row and column per lane");
   // Two-stage 1NN filter (rows and columns, long chunks): every pair is
-  // first compared once with mt = min(T_lo, column threshold), T_lo the
-  // lowest row threshold of the block; only rows where some pair passes run
-  // the exact row and column tests.  With converged thresholds a few percent
-  // of rows pass the first stage, so a pair costs one compare instead of
-  // two.  The FP64 form keeps each column's threshold as an integer image
-  // (one register, one DPP move) next to the double mt.
+  // first tested once against min(T_lo, column threshold), T_lo the lowest
+  // row threshold of the block; only rows where some pair passes run the
+  // exact row and column tests.  The first stage works in row-scaled units:
+  // s = cov * in_row against mt = min(T_lo, column threshold) / in_col
+  // (a per-column bound, made when the column enters the band or a block
+  // starts, that travels with the column), so a pair costs one multiply and
+  // one compare; the exact correlation s * in_col is formed only on rows that
+  // pass.  With converged thresholds a few percent of rows do.  The FP64
+  // form keeps each column's threshold as an integer image (one register,
+  // one DPP move) for its exact column test.
   static constexpr bool kTwoStage = KIND == kNN && !FRESH && COLS && ROWS;
   static constexpr bool kIntCol = kTwoStage && sizeof(DT) == 8;
   // the block's row bound: FP32 the threshold itself, FP64 its enc_thr image
@@ -261,9 +287,9 @@ struct Sweep {
   // row bound and the column's threshold.
   __device__ __forceinline__ void refresh_mt(Slot<DT> &s) const {
     if constexpr (kIntCol)
-      s.mt = mt_of(tlo_i, s.F);
+      s.mt = stage1_bound(tlo_i, s.F, s.in);
     else
-      s.mt = min_thr(tlo_f, s.t);
+      s.mt = stage1_bound32(tlo_f, s.t, s.in);
   }
 
   // Terms staged a block ahead: lane l fetches row ib+l (the rows of the
@@ -311,8 +337,10 @@ struct Sweep {
       r.t = canon_thr(r.t);
       c.t = canon_thr(c.t);
     }
-    s_row[lane] = r;
-    s_col[lane] = c;
+    if constexpr (!kTwoStage) {
+      s_row[lane] = r;
+      s_col[lane] = c;
+    }
     if constexpr (kTwoStage) {
       // T_lo: the lowest threshold of the block's rows (lanes < kRows stage
       // them; rows that can never match do not count)
@@ -325,11 +353,18 @@ struct Sweep {
           v = o < v ? o : v;
         }
         tlo_i = __builtin_amdgcn_readfirstlane(v);
+        // the entering column's first-stage bound travels in its record's t
+        c.t = stage1_bound(tlo_i, c.F, c.in);
+        s_row[lane] = r;
+        s_col[lane] = c;
       } else {
         float v = mine ? (float)r.t : __builtin_inff();
 #pragma unroll
         for (int off = kRows / 2; off > 0; off >>= 1) v = min_thr(v, __shfl_xor(v, off));
         tlo_f = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+        c.b = stage1_bound32(tlo_f, c.t, c.in);
+        s_row[lane] = r;
+        s_col[lane] = c;
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -770,11 +805,39 @@ struct Sweep {
   // Row i = i_b + r of a block whose first row satisfies (i_b + D0) % K == 0,
   // so diagonal k of every lane crosses the column in slot (r + k) % K.
   // r: slot cycle position (i - i_b) mod K; rr: row of the block (i - i_b).
+  // Row i of a two-stage 1NN kernel (see kTwoStage).
+  template <bool MASKED>
+  __device__ __forceinline__ void row_two_stage(int r, int rr, int64_t i, DT rin,
+                                                DT trow) {
+    DT c[K];
+    // first stage on s = cov * in_row against the columns' bounds; the
+    // exact correlation s * in_col only where some pair passes
+    uint64_t hit1 = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * rin;
+      if (MASKED && i + d0 + k >= a.nX) c[k] = nan_of<DT>();
+      hit1 |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].mt);
+    }
+    if (hit1) {
+      uint64_t hit = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        c[k] = c[k] * S[(r + k) % K].in;
+        hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
+        hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
+      }
+      if (hit) {
+        nn_row_update(c, i, rr, trow);
+        nn_col_update(r, rr, c, i);
+      }
+    }
+  }
+
+  // Row i of every other kernel: the correlations, then the kind's tests.
   template <bool MASKED, bool FAST>
-  __device__ __forceinline__ void row(int r, int rr, int64_t i,
-                                      const Terms<DT> *s_row) {
-    const Terms<DT> rv = s_row[rr];
-    const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
+  __device__ __forceinline__ void row_tests(int r, int rr, int64_t i, DT rin,
+                                            DT trow) {
     DT c[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -793,23 +856,6 @@ struct Sweep {
       }
       if (ROWS && hit_r) nn_row_update(c, i, rr, trow);
       if (COLS && hit_c) nn_col_update(r, rr, c, i);
-    } else if (kTwoStage) {
-      uint64_t hit1 = 0;
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        hit1 |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].mt);
-      if (hit1) {
-        uint64_t hit = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
-          hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
-        }
-        if (hit) {
-          nn_row_update(c, i, rr, trow);
-          nn_col_update(r, rr, c, i);
-        }
-      }
     } else if (kThr) {
       // One compare per test written straight into a wave mask (keeps the
       // compiler from merging the two tests into a NaN-canonicalising min).
@@ -858,6 +904,17 @@ struct Sweep {
         }
       }
     }
+  }
+
+  template <bool MASKED, bool FAST>
+  __device__ __forceinline__ void row(int r, int rr, int64_t i,
+                                      const Terms<DT> *s_row) {
+    const Terms<DT> rv = s_row[rr];
+    const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
+    if constexpr (kTwoStage)
+      row_two_stage<MASKED>(r, rr, i, rin, trow);
+    else
+      row_tests<MASKED, FAST>(r, rr, i, rin, trow);
     // advance the recurrence to row i+1
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -882,11 +939,13 @@ struct Sweep {
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
     s.in = shl1(s.in, e.in);
-    if constexpr (kIntCol)
+    if constexpr (kIntCol) {
       s.F = shl1(s.F, e.F);
-    else if (kThr && COLS)
+      s.mt = shl1(s.mt, e.t);
+    } else if (kThr && COLS) {
       s.t = shl1(s.t, e.t);
-    if constexpr (kTwoStage) refresh_mt(s);
+      if constexpr (kTwoStage) s.mt = shl1(s.mt, e.b);
+    }
   }
 
   // Flushes what a lane still holds when its segment ends after row i.

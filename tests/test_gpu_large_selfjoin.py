@@ -40,8 +40,18 @@ def series():
     return np.cumsum(steps, dtype=np.float32).astype(np.float64)
 
 
+_CAPSYS = None
+
+
 def log(msg, t0):
-    print(f"[n=2^25 m=4096] {msg} ({time.perf_counter() - t0:.1f} s)", flush=True)
+    """Progress to the terminal even under pytest's output capture (each
+    phase of this test runs for a minute or more)."""
+    line = f"[n=2^25 m=4096] {msg} ({time.perf_counter() - t0:.1f} s)"
+    if _CAPSYS is not None:
+        with _CAPSYS.disabled():
+            print(line, flush=True)
+    else:
+        print(line, flush=True)
 
 
 def fft_row(a, i, inorm, e):
@@ -61,7 +71,9 @@ def fft_row(a, i, inorm, e):
 
 
 @pytest.mark.timeout(1500)
-def test_large_selfjoin_mixed_profile_and_summary():
+def test_large_selfjoin_mixed_profile_and_summary(capsys):
+    global _CAPSYS
+    _CAPSYS = capsys
     import torch
     from scamp_amd import tuning
     t0 = time.perf_counter()
