@@ -51,9 +51,11 @@ namespace sweep {
 
 constexpr int K = kDiagsPerLane;
 // u-samples per LDS stage of a seed / re-anchor: the FP32 form has its own
-// staging; the FP64 form stages through its row and column term arrays
-// (idle at a segment start, the only place it seeds), which hold 640 doubles
-constexpr int kAnchorChunk = 256;
+// staging, sized (with the staged terms) to 10 KB, which lets 16 one-wave
+// workgroups share a CU's 160 KB of LDS (4 waves per SIMD); the FP64 form
+// stages through its row and column term arrays (idle at a segment start,
+// the only place it seeds), which hold 640 doubles
+constexpr int kAnchorChunk = 184;
 constexpr int kAnchorChunk64 = 64;
 
 template <typename DT>
@@ -75,12 +77,16 @@ struct Terms {
   DT df, dg, in;
   DT t;       // 1NN: threshold of the row's / column's current best
   DT b;       // FP32 two-stage filter: the column's first-stage bound
+  DT s1;      // two-stage filter, rows: the first stage's scale (in, or 0
+              // for a row that can never match)
 };
 template <>
 struct Terms<double> {
   double df, dg, in;
   double t;
   int32_t F;  // two-stage filter: enc_thr of the column's threshold
+  double s1;  // two-stage filter, rows: the first stage's scale (in, or 0
+              // for a row that can never match)
 };
 
 // Integer image of an FP64 threshold t for the two-stage filter: for c >= 0,
@@ -112,25 +118,44 @@ __device__ __forceinline__ float min_thr(float a, float b) {
 __device__ __forceinline__ double mt_of(int32_t tlo, int32_t f) {
   return __hiloint2double(tlo < f ? tlo : f, 0);
 }
-// The FP64 first stage in row-scaled units: a pair passes when
-// s = cov * in_row >= B, B = mt_of(tlo, f) / in_col less a relative 2^-30,
-// which (s * in_col >= threshold implies s >= B under FP64 rounding, with
-// the reciprocal good to far better than 2^-31 after one Newton step) keeps
-// the stage conservative.  A column that can never match has in_col = NaN,
-// so B = NaN and no pair passes.
+// The FP64 first stage in row-scaled units: a pair passes when the exact
+// product cov * in_row is at least B, B = mt_of(tlo, f) / in_col less a
+// relative 2^-30 and an absolute 2^-1000, which (s * in_col >= threshold
+// implies s >= B under FP64 rounding, with the reciprocal good to far better
+// than 2^-31 after one Newton step) keeps the stage conservative.  The test
+// is the sign of fma(cov, in_row, -B), so B is never NaN: a column that can
+// never match (in_col = NaN) gets B = +inf, which no pair reaches.
 __device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, double in_col) {
   double r = __builtin_amdgcn_rcp(in_col);
   r = fma(r, fma(-in_col, r, 1.0), r);
   const double b = mt_of(tlo, f) * r;
-  return fma(-fabs(b), 0x1p-30, b);
+  const double lowered = fma(-fabs(b), 0x1p-30, b) - 0x1p-1000;
+  return in_col == in_col ? lowered : __builtin_inf();
 }
+// Two-stage kernels: the covariance an invalid diagonal (outside the pass,
+// at a band's edge) carries instead of NaN, so that its first-stage
+// difference is hugely negative rather than NaN (whose sign bit is not
+// defined); the exact stage masks such diagonals to NaN.  Far below any
+// covariance, and times any inverse norm (< 2^22) still finite.
+template <typename DT>
+constexpr DT far_cov();
+template <>
+constexpr double far_cov<double>() { return -0x1p1000; }
+template <>
+constexpr float far_cov<float>() { return -0x1p100f; }
+
+// The integer whose sign is the sign of x (FP64: the high word).
+__device__ __forceinline__ int32_t sign_word(double x) { return __double2hiint(x); }
+__device__ __forceinline__ int32_t sign_word(float x) { return __float_as_int(x); }
 
 // The FP32 first stage in row-scaled units (as stage1_bound): B =
-// min(tlo, t) / in_col less a relative 2^-16, far above the FP32 rounding of
-// the product and of the hardware reciprocal.
+// min(tlo, t) / in_col less a relative 2^-16 (far above the FP32 rounding of
+// the product and of the hardware reciprocal) and an absolute 2^-100; +inf
+// for a column that can never match.
 __device__ __forceinline__ float stage1_bound32(float tlo, float t, float in_col) {
   const float b = min_thr(tlo, t) * __builtin_amdgcn_rcpf(in_col);
-  return fmaf(-fabsf(b), 0x1p-16f, b);
+  const float lowered = fmaf(-fabsf(b), 0x1p-16f, b) - 0x1p-100f;
+  return in_col == in_col ? lowered : __builtin_inff();
 }
 
 template <typename DT>
@@ -222,6 +247,17 @@ struct Sweep {
   // one DPP move) for its exact column test.
   static constexpr bool kTwoStage = KIND == kNN && !FRESH && COLS && ROWS;
   static constexpr bool kIntCol = kTwoStage && sizeof(DT) == 8;
+  // The first stage is tested by sign: d_k = fma(cov_k, in_row, -B_k) for
+  // the lane's 8 pairs, and a row passes when the largest high word (FP64)
+  // or bit pattern (FP32) of the d_k is >= 0 as an integer, i.e. some
+  // d_k >= 0: 8 FMAs and a max3 tree per row in place of 8 multiplies and 8
+  // compares, each of which would hand a mask to the scalar unit.  No d_k
+  // may be NaN (its sign bit is not defined): invalid diagonals carry
+  // far_cov() instead of NaN (the exact stage masks them with vmask), a row
+  // that can never match scales its first stage by 0 (Terms::s1) while its
+  // exact stage keeps the NaN inverse norm, and a column that can never
+  // match has B = +inf.
+  uint32_t vmask = 0;  // two-stage kernels: the lane's valid diagonals
   // the block's row bound: FP32 the threshold itself, FP64 its enc_thr image
   DT tlo_f = (DT)0;
   int32_t tlo_i = 0;
@@ -355,6 +391,7 @@ struct Sweep {
         tlo_i = __builtin_amdgcn_readfirstlane(v);
         // the entering column's first-stage bound travels in its record's t
         c.t = stage1_bound(tlo_i, c.F, c.in);
+        r.s1 = r.in == r.in ? r.in : 0.0;
         s_row[lane] = r;
         s_col[lane] = c;
       } else {
@@ -363,6 +400,7 @@ struct Sweep {
         for (int off = kRows / 2; off > 0; off >>= 1) v = min_thr(v, __shfl_xor(v, off));
         tlo_f = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
         c.b = stage1_bound32(tlo_f, c.t, c.in);
+        r.s1 = r.in == r.in ? r.in : 0.0f;
         s_row[lane] = r;
         s_col[lane] = c;
       }
@@ -430,14 +468,22 @@ struct Sweep {
     if (kTable && lane < kRows) s_rt[lane] = 0;
     if (kColTable)
       for (int q = lane; q < K * kColStride; q += kWave) s_ct[q] = 0;
+    if (kTwoStage) vmask = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
       const double v = acc[k] - (a.muX[i + d] - ref) * sy;
-      if constexpr (kComp) {
+      if constexpr (kTwoStage) vmask |= (diag_valid(d) ? 1u : 0u) << k;
+      if constexpr (kComp && kTwoStage) {
+        const DT h = (DT)v;
+        hi[k] = diag_valid(d) ? h : far_cov<DT>();
+        cov[k] = diag_valid(d) ? (DT)(v - (double)h) : (DT)0;
+      } else if constexpr (kComp) {
         const DT h = (DT)v;
         hi[k] = diag_valid(d) ? h : nan_of<DT>();
         cov[k] = (DT)(v - (double)h);
+      } else if constexpr (kTwoStage) {
+        cov[k] = diag_valid(d) ? (DT)v : far_cov<DT>();
       } else {
         cov[k] = diag_valid(d) ? (DT)v : nan_of<DT>();
       }
@@ -805,32 +851,36 @@ struct Sweep {
   // Row i = i_b + r of a block whose first row satisfies (i_b + D0) % K == 0,
   // so diagonal k of every lane crosses the column in slot (r + k) % K.
   // r: slot cycle position (i - i_b) mod K; rr: row of the block (i - i_b).
-  // Row i of a two-stage 1NN kernel (see kTwoStage).
+  // Row i of a two-stage 1NN kernel (see kTwoStage); rs1 is the row's
+  // first-stage scale (Terms::s1).
   template <bool MASKED>
   __device__ __forceinline__ void row_two_stage(int r, int rr, int64_t i, DT rin,
-                                                DT trow) {
+                                                DT rs1, DT trow) {
+    // first stage: some d_k = cov_k * in_row - B_k >= 0, by the sign of the
+    // largest integer image of the d_k (a max3 tree)
+    int32_t h[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      h[k] = sign_word(fma(kComp ? hi[k] + cov[k] : cov[k], rs1, -S[(r + k) % K].mt));
+    static_assert(K == 8, "the max tree below takes 8 differences");
+    const int32_t m0 = max(max(h[0], h[1]), h[2]);
+    const int32_t m1 = max(max(h[3], h[4]), h[5]);
+    const int32_t mx = max(max(h[6], h[7]), max(m0, m1));
+    if (__builtin_amdgcn_ballot_w64(mx >= 0) == 0) return;
+    // exact stage: the correlations (cov * in_row) * in_col, NaN where the
+    // diagonal is outside the pass or the column past the series
     DT c[K];
-    // first stage on s = cov * in_row against the columns' bounds; the
-    // exact correlation s * in_col only where some pair passes
-    uint64_t hit1 = 0;
+    uint64_t hit = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * rin;
-      if (MASKED && i + d0 + k >= a.nX) c[k] = nan_of<DT>();
-      hit1 |= __builtin_amdgcn_ballot_w64(c[k] >= S[(r + k) % K].mt);
+      c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * rin * S[(r + k) % K].in;
+      if (!((vmask >> k) & 1u) || (MASKED && i + d0 + k >= a.nX)) c[k] = nan_of<DT>();
+      hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
+      hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
     }
-    if (hit1) {
-      uint64_t hit = 0;
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        c[k] = c[k] * S[(r + k) % K].in;
-        hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
-        hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
-      }
-      if (hit) {
-        nn_row_update(c, i, rr, trow);
-        nn_col_update(r, rr, c, i);
-      }
+    if (hit) {
+      nn_row_update(c, i, rr, trow);
+      nn_col_update(r, rr, c, i);
     }
   }
 
@@ -912,7 +962,7 @@ struct Sweep {
     const Terms<DT> rv = s_row[rr];
     const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
     if constexpr (kTwoStage)
-      row_two_stage<MASKED>(r, rr, i, rin, trow);
+      row_two_stage<MASKED>(r, rr, i, rin, rv.s1, trow);
     else
       row_tests<MASKED, FAST>(r, rr, i, rin, trow);
     // advance the recurrence to row i+1
@@ -1170,6 +1220,7 @@ template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH>
 __global__ __launch_bounds__(kWave)
 __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<DT, KIND>())))
 void sweep_kernel(SweepArgs a) {
+  using SW = Sweep<DT, KIND, COLS, ROWS, FRESH>;
   __shared__ Terms<DT> s_terms[2 * kWave];  // staged row / column terms
   Terms<DT> *const s_row = s_terms;
   Terms<DT> *const s_col = s_terms + kWave;
@@ -1188,7 +1239,6 @@ void sweep_kernel(SweepArgs a) {
   // anchor staging, which is only in use between blocks (anchor() clears
   // them again when it is done).
   constexpr bool kTable = KIND == kNN && FRESH;
-  using SW = Sweep<DT, KIND, COLS, ROWS, FRESH>;
   __shared__ uint64_t s_rt_own[kTable && !kF32 ? SW::kRows : 1];
   __shared__ uint64_t s_ct_own[SW::kColTable && !kF32 ? kDiagsPerLane * SW::kColStride : 1];
   static_assert(kDiagsPerLane * SW::kColStride <= kBand + kAnchorChunk,
