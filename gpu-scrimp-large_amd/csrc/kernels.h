@@ -59,9 +59,13 @@ hipError_t launch_profile_in(const void *src, bool mp, uint64_t *keys,
 hipError_t launch_profile_out(const uint64_t *keys, int64_t kstride, void *dst,
                               bool mp, int64_t n, int64_t batch, hipStream_t s);
 // Self-join 1NN warm start: each subsequence's entries take the best of its
-// correlations with the 8 windows just past the exclusion zone e on either
-// side (forward partners into keyRow, backward ones into keyCol; either may
-// be null; equal pointers merge both).  batch series, stride entries apart.
+// correlations with the kWarmLags windows just past the exclusion zone e on
+// either side (forward partners into keyRow, backward ones into keyCol;
+// either may be null; equal pointers merge both).  Every pair on diagonals
+// [e, e + kWarmLags) is merged into both of its entries, so the sweep that
+// follows starts at diagonal e + kWarmLags.  batch series, stride entries
+// apart.
+constexpr int64_t kWarmLags = 8;
 hipError_t launch_warm_profile(const double *clean, const double *mean,
                                const double *inorm, int64_t ns, int64_t m,
                                int64_t e, int64_t batch, int64_t stride,
@@ -120,6 +124,20 @@ hipError_t launch_knn_to_matches(uint64_t *lists, int64_t n, int k, uint64_t ini
 // ---- the diagonal sweep (sweep_kernel.hip) --------------------------------
 enum SweepKind { kNN = 0, kSum = 1, kMatrix = 2, kMatrixSmall = 3, kKnn = 4 };
 
+// Chunk length of a band of rows0 rows in a chunked pass: R for the longest
+// band (ref_rows rows), and shorter in proportion for shorter bands (never
+// below rmin, a multiple of 16), so that every band has about as many items.
+// Items run band by band, longest bands first, so the items that start last
+// are short and the launch does not end on a few long items while the rest
+// of the GPU idles.
+__host__ __device__ inline int64_t band_chunk_rows(int64_t R, int64_t rmin,
+                                                   int64_t ref_rows, int64_t rows0) {
+  if (rows0 >= ref_rows) return R;
+  int64_t r = (R * rows0 + ref_rows - 1) / ref_rows;
+  r = (r + 15) / 16 * 16;
+  return r < rmin ? rmin : (r > R ? R : r);
+}
+
 struct SweepArgs {
   // column side X / row side Y update terms and inverse norms (FP64 or FP32,
   // padded by kColumnPad entries)
@@ -128,10 +146,11 @@ struct SweepArgs {
   // clean series and means (the in-kernel FP64 seeds and re-anchors)
   const double *TX, *TY, *muX, *muY;
   // work items, taken in order through `counter` and decoded in the kernel
-  // (plan_item): nitems = bands x nbatch x nchunks; chunk_rows is the row
-  // length R of a chunk when nchunks > 1
+  // (plan_item): nitems = bands x nbatch x nchunks; when nchunks > 1 a band
+  // of rows0 rows is cut into chunks of band_chunk_rows(chunk_rows,
+  // chunk_min_rows, chunk_ref_rows, rows0) rows
   int64_t nitems;
-  int64_t nchunks, chunk_rows;
+  int64_t nchunks, chunk_rows, chunk_min_rows, chunk_ref_rows;
   int64_t nX, nY;        // subsequence counts of X and Y
   int64_t d_lo, d_hi;    // diagonal range of this pass
   int64_t d_base;        // d_lo rounded down to a multiple of 8: band b

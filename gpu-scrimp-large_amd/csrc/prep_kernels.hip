@@ -375,7 +375,7 @@ inline unsigned blocks_for(int64_t n, int bs) {
 // with atomicMax too, as both may land in one profile.  One series per
 // blockIdx.y (stride entries apart); samples at or past n read as 0 and meet
 // x = 0 or an invalid partner.
-constexpr int kWarm = 8;
+constexpr int kWarm = (int)kWarmLags;
 constexpr int kWarmBlock = 256;
 constexpr int kWarmChunk = 256;
 constexpr int kWarmStage = kWarmBlock + kWarmChunk + 2 * kWarm;

@@ -328,8 +328,10 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
   // r0 + j * R: r0 is a hashed multiple of 8 below rfull, so that items
   // running at the same time work on different rows' profile entries; R is
   // unbounded when the pass has enough bands to fill the GPU, otherwise
-  // short enough that the items do, and never below max(512, 4m) rows so the
-  // segment seeds stay cheap next to the rows they start.
+  // short enough that the items do, and for the longest band never below
+  // max(512, 4m) rows so the segment seeds stay cheap next to the rows they
+  // start; shorter bands take proportionally shorter chunks
+  // (band_chunk_rows).
   auto band_rows = [&](int64_t b, int64_t *rfull) {
     const int64_t D0 = d_base + b * kBand;
     const int64_t dfirst = std::max(D0, p.d_lo);
@@ -353,7 +355,15 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
   }
   const bool chunked = R != std::numeric_limits<int64_t>::max();
   int64_t nchunks = 1;
+  // shorter bands take proportionally shorter chunks, down to rmin rows
+  // (at least the window, so a chunk's seed stays cheap next to its rows)
+  const int64_t rmin = std::min<int64_t>(R, (std::max<int64_t>(128, m) + 15) / 16 * 16);
+  int64_t ref_rows = 0;
   if (chunked) {
+    for (int64_t b = 0; b < nbands; ++b) {
+      int64_t rfull;
+      ref_rows = std::max(ref_rows, band_rows(b, &rfull));
+    }
     // the largest chunk count of any band (the kernel skips the items a
     // shorter band does not have); the same cuts as plan_item
     for (int64_t b = 0; b < nbands; ++b) {
@@ -363,9 +373,10 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
       const int64_t slots = (rfull - 1) / kDiagsPerLane + 1;
       const int64_t r0 = (int64_t)(mix64((uint64_t)b + 0x9e3779b97f4a7c15ULL) %
                                    (uint64_t)slots) * kDiagsPerLane;
-      int64_t first = r0 % R;
-      if (first == 0) first = R;
-      const int64_t n = (first < rows0 ? (rows0 - 1 - first) / R + 1 : 0) + 1;
+      const int64_t Rb = band_chunk_rows(R, rmin, ref_rows, rows0);
+      int64_t first = r0 % Rb;
+      if (first == 0) first = Rb;
+      const int64_t n = (first < rows0 ? (rows0 - 1 - first) / Rb + 1 : 0) + 1;
       nchunks = std::max(nchunks, n);
     }
   }
@@ -386,6 +397,7 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
   }
   a.TX = p.X->clean; a.TY = p.Y->clean; a.muX = p.X->mean; a.muY = p.Y->mean;
   a.nitems = nitems; a.nchunks = nchunks; a.chunk_rows = chunked ? R : 0;
+  a.chunk_min_rows = rmin; a.chunk_ref_rows = ref_rows;
   a.nX = nX; a.nY = nY; a.d_lo = p.d_lo; a.d_hi = p.d_hi; a.d_base = d_base;
   a.m = m; a.anchor = anchor;
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
@@ -630,14 +642,18 @@ int join_device(scamp_hip_ctx *ctx, Arena &ar, const double *d_a, int64_t na,
   // the windows just past the exclusion zone (pairs of the join, so the
   // result is the same; the sweep's thresholds then start near their final
   // values, which short series would otherwise spend much of their rows on).
-  if (self && kind == kNN && (cols_out || rows_out))
+  // Those diagonals are then done: were the sweep to evaluate them again,
+  // every row of its first band would meet its warm-start match exactly at
+  // its threshold and take the merge path.
+  const bool warm = self && kind == kNN && (cols_out || rows_out);
+  if (warm)
     HIP_CHECK(launch_warm_profile(A.clean, A.mean, A.inorm, nsA, m, e, 1, 0,
                                   cols_out ? keyA : nullptr,
                                   rows_out ? (sep_rows ? keyB : keyA) : nullptr, s));
   Pass passes[kMaxPasses];
   int npass = 0;
   if (self) {
-    const Ranges r = cut(e, nsA, 0, 0);
+    const Ranges r = cut(warm ? e + kWarmLags : e, nsA, 0, 0);
     for (int i = 0; i < r.count; ++i) {
       Pass ps{&A, &A, r.lo[i], r.hi[i], cols_out, rows_out, keyA,
               sep_rows ? keyB : keyA, sumA, sep_rows ? sumB : sumA,
@@ -826,7 +842,8 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
                                 s));
   uint64_t *counters = ar.alloc<uint64_t>(kMaxPasses);
   float sweep_ms = 0;
-  const Ranges r = cut(m / 4, ns, 0, 0);
+  // the warm start has done diagonals [m/4, m/4 + kWarmLags)
+  const Ranges r = cut(m / 4 + ((cols || rows) ? kWarmLags : 0), ns, 0, 0);
   for (int i = 0; i < r.count; ++i) {
     Pass ps{&A, &A, r.lo[i], r.hi[i], cols, rows, key, key,
             nullptr, nullptr, nullptr, nullptr};

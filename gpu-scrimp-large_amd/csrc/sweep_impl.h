@@ -1173,9 +1173,10 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 // spans rows [0, rows0) and starts at a hashed row r0 (a multiple of 8 below
 // rfull), so that bands in flight sit at unrelated rows' profile entries:
 //  * unchunked (nchunks == 1): one item, rows [r0, rows0) then [0, r0);
-//  * chunked: the rows are cut at r0 + j R, and the band's item j is its
-//    j-th chunk counting from the one that starts at r0 (wrapping round to
-//    row 0); items beyond the band's chunk count are empty.
+//  * chunked: the rows are cut at r0 + j R (R from band_chunk_rows: shorter
+//    bands, which run later, take shorter chunks), and the band's item j is
+//    its j-th chunk counting from the one that starts at r0 (wrapping round
+//    to row 0); items beyond the band's chunk count are empty.
 // Returns false for an empty item.
 __device__ __forceinline__ bool plan_item(const SweepArgs &a, unsigned int idx,
                                           ItemRows &g) {
@@ -1201,7 +1202,7 @@ __device__ __forceinline__ bool plan_item(const SweepArgs &a, unsigned int idx,
     g.re2 = r0;
     return true;
   }
-  const int64_t R = a.chunk_rows;
+  const int64_t R = band_chunk_rows(a.chunk_rows, a.chunk_min_rows, a.chunk_ref_rows, rows0);
   int64_t first = r0 % R;  // the lowest cut above row 0
   if (first == 0) first = R;
   const int64_t cuts = first < rows0 ? (rows0 - 1 - first) / R + 1 : 0;
