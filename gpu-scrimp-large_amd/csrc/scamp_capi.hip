@@ -41,9 +41,16 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 // contexts, on the same device or not, run independently.
 struct scamp_hip_ctx {
   int device = 0;
+  // the stream the context's work goes on: its own (`own`), or the caller's
+  // after scamp_hip_ctx_set_stream, which also makes the device-buffer calls
+  // return once their work is queued (`async`)
   hipStream_t stream = nullptr;
+  hipStream_t own = nullptr;
+  bool async = false;
   int num_cus = 0;
   float total_ms = 0, prep_ms = 0, sweep_ms = 0;
+  // async: the last join's events are recorded but not yet read
+  bool timing_pending = false;
   // the precision the last join ran in, and the entries whose FP32 sweep
   // value failed the mixed form's audit (then the join ran again in FP64)
   int last_precision = 0;
@@ -164,13 +171,21 @@ struct DevArena {
     ptrs.emplace_back(got, p);
     return static_cast<T *>(p);
   }
+  // Blocks go back to the cache for the context's next call, which runs
+  // after this one on the same stream; an asynchronous context waits for its
+  // stream before it frees one that does not fit the cache.
   ~DevArena() {
+    bool waited = !ctx->async;
     for (auto &b : ptrs) {
       if (b.first <= scamp_hip_ctx::kCacheBlock &&
           ctx->cached + b.first <= scamp_hip_ctx::kCacheTotal) {
         ctx->cache.push_back(b);
         ctx->cached += b.first;
       } else {
+        if (!waited) {
+          (void)hipStreamSynchronize(ctx->stream);
+          waited = true;
+        }
         (void)hipFree(b.second);
       }
     }
@@ -284,6 +299,41 @@ uint64_t mix64(uint64_t x) {
   x *= 0xc4ceb9fe1a85ec53ULL;
   x ^= x >> 33;
   return x;
+}
+
+// End of a join: a blocking context waits for the end event and records
+// the join's times; an asynchronous one (scamp_hip_ctx_set_stream) records
+// the event and leaves reading the times to scamp_hip_last_timing.  Returns
+// whether the times are known.  ev[0] / ev[2] bracket the join, ev[3] /
+// ev[4] its sweep when asynchronous (a blocking join adds up its passes'
+// sweep times as they finish).
+bool finish_join(scamp_hip_ctx *ctx, float sweep_ms) {
+  HIP_CHECK(hipEventRecord(ctx->ev[2], ctx->stream));
+  if (ctx->async) {
+    ctx->timing_pending = true;
+    return false;
+  }
+  HIP_CHECK(hipEventSynchronize(ctx->ev[2]));
+  float total = 0;
+  HIP_CHECK(hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[2]));
+  ctx->total_ms = total;
+  ctx->sweep_ms = sweep_ms;
+  ctx->prep_ms = total - sweep_ms;
+  ctx->timing_pending = false;
+  return true;
+}
+
+// Reads the times an asynchronous join left pending (waits for its end).
+void resolve_timing(scamp_hip_ctx *ctx) {
+  if (!ctx->timing_pending) return;
+  HIP_CHECK(hipEventSynchronize(ctx->ev[2]));
+  float total = 0, sweep = 0;
+  HIP_CHECK(hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[2]));
+  HIP_CHECK(hipEventElapsedTime(&sweep, ctx->ev[3], ctx->ev[4]));
+  ctx->total_ms = total;
+  ctx->sweep_ms = sweep;
+  ctx->prep_ms = total - sweep;
+  ctx->timing_pending = false;
 }
 
 struct Pass {
@@ -690,9 +740,11 @@ int join_device(scamp_hip_ctx *ctx, Arena &ar, const double *d_a, int64_t na,
     }
   }
   float sweep_ms = 0;
+  if (ctx->async) HIP_CHECK(hipEventRecord(ctx->ev[3], s));
   for (int i = 0; i < npass; ++i)
     run_pass(ctx, bf.counters + i, passes[i], m, fp32, kind, mat, W, rpc, cpc,
-             p->distance_threshold, anchor, knn_k, &sweep_ms);
+             p->distance_threshold, anchor, knn_k, ctx->async ? nullptr : &sweep_ms);
+  if (ctx->async) HIP_CHECK(hipEventRecord(ctx->ev[4], s));
 
   if (refine) {
     // profile A holds A's windows matched in B (self: in A); profile B the
@@ -762,19 +814,12 @@ int join_device(scamp_hip_ctx *ctx, Arena &ar, const double *d_a, int64_t na,
     HIP_CHECK(hipMemcpyAsync(d_prof_a, mat, sizeof(float) * H * W,
                              hipMemcpyDeviceToDevice, s));
   }
-  HIP_CHECK(hipEventRecord(ev_end, s));
-  HIP_CHECK(hipEventSynchronize(ev_end));
-  float total = 0;
-  HIP_CHECK(hipEventElapsedTime(&total, ev_start, ev_end));
-  ctx->total_ms = total;
-  ctx->sweep_ms = sweep_ms;
-  ctx->prep_ms = total - sweep_ms;
   ctx->last_precision = prec;
   if (!fp32) ctx->last_drift = 0;
-  if (!p->silent_mode)
+  if (finish_join(ctx, sweep_ms) && !p->silent_mode)
     std::printf("scamp_hip: join of %lld x %lld subsequences in %.3f ms "
                 "(sweep %.3f ms)\n",
-                (long long)nsA, (long long)nsB, total, sweep_ms);
+                (long long)nsA, (long long)nsB, ctx->total_ms, sweep_ms);
   return SCAMP_HIP_OK;
 }
 
@@ -844,12 +889,14 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   float sweep_ms = 0;
   // the warm start has done diagonals [m/4, m/4 + kWarmLags)
   const Ranges r = cut(m / 4 + ((cols || rows) ? kWarmLags : 0), ns, 0, 0);
+  if (ctx->async) HIP_CHECK(hipEventRecord(ctx->ev[3], s));
   for (int i = 0; i < r.count; ++i) {
     Pass ps{&A, &A, r.lo[i], r.hi[i], cols, rows, key, key,
             nullptr, nullptr, nullptr, nullptr};
     run_pass(ctx, counters + i, ps, m, fp32, kNN, nullptr, 0, 1, 1, 0.0, anchor,
-             0, &sweep_ms, batch, seg, ns);
+             0, ctx->async ? nullptr : &sweep_ms, batch, seg, ns);
   }
+  if (ctx->async) HIP_CHECK(hipEventRecord(ctx->ev[4], s));
   if (refine) {
     HIP_CHECK(launch_refine(key, prior, ns, A.clean, A.mean, A.inorm, A.df, A.dg,
                             ns, A.clean, A.mean, A.inorm, A.df, A.dg, m,
@@ -876,19 +923,12 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   }
 
   HIP_CHECK(launch_profile_out(key, seg, d_prof, mp, ns, batch, s));
-  HIP_CHECK(hipEventRecord(ev_end, s));
-  HIP_CHECK(hipEventSynchronize(ev_end));
-  float total = 0;
-  HIP_CHECK(hipEventElapsedTime(&total, ev_start, ev_end));
-  ctx->total_ms = total;
-  ctx->sweep_ms = sweep_ms;
-  ctx->prep_ms = total - sweep_ms;
   ctx->last_precision = prec;
   if (!fp32) ctx->last_drift = 0;
-  if (!p->silent_mode)
+  if (finish_join(ctx, sweep_ms) && !p->silent_mode)
     std::printf("scamp_hip: %lld self-joins of %lld subsequences in %.3f ms "
                 "(sweep %.3f ms)\n",
-                (long long)batch, (long long)ns, total, sweep_ms);
+                (long long)batch, (long long)ns, ctx->total_ms, sweep_ms);
   return SCAMP_HIP_OK;
 }
 
@@ -903,6 +943,7 @@ int with_fp64_rerun(scamp_hip_ctx *ctx, const scamp_hip_params *p, Run run) {
   scamp_hip_params p64 = *p;
   p64.precision = SCAMP_HIP_DOUBLE;
   rc = run(&p64);
+  resolve_timing(ctx);  // an asynchronous rerun's times (reruns are rare)
   ctx->total_ms += t0;
   ctx->sweep_ms += s0;
   ctx->prep_ms = ctx->total_ms - ctx->sweep_ms;
@@ -945,7 +986,8 @@ int scamp_hip_ctx_create(int device, scamp_hip_ctx **out) {
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     ctx->num_cus = prop.multiProcessorCount;
-    HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+    ctx->stream = ctx->own;
     for (hipEvent_t &e : ctx->ev) HIP_CHECK(hipEventCreate(&e));
   } catch (const HipFailure &f) {
     scamp_hip_ctx_destroy(ctx);
@@ -976,8 +1018,28 @@ int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx) {
   for (auto &b : ctx->cache) (void)hipFree(b.second);
   for (hipEvent_t e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->own) (void)hipStreamDestroy(ctx->own);
   delete ctx;
+  return SCAMP_HIP_OK;
+}
+
+int scamp_hip_ctx_set_stream(scamp_hip_ctx *ctx, void *stream) {
+  if (!ctx) return fail(SCAMP_HIP_INVALID_ARGS, "null context");
+  try {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t want = stream ? static_cast<hipStream_t>(stream) : ctx->own;
+    if (want != ctx->stream) {
+      // the cached blocks may still be in use by work on the old stream
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      resolve_timing(ctx);
+      ctx->stream = want;
+    }
+    ctx->async = stream != nullptr;
+  } catch (const HipFailure &f) {
+    return fail(SCAMP_HIP_DEVICE_ERROR,
+                std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
   return SCAMP_HIP_OK;
 }
 
@@ -1052,11 +1114,15 @@ int scamp_hip_join_device_noalloc(scamp_hip_ctx *ctx, const double *d_a,
     std::lock_guard<std::mutex> dev_lock(g_scratch[ctx->device].mu);
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_CHECK(hipSetDevice(ctx->device));
-    return with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
+    const int rc = with_fp64_rerun(ctx, p, [&](const scamp_hip_params *q) {
       ScratchArena ar(ctx->scratch, kScratchBytes);
       return join_device(ctx, ar, d_a, (int64_t)na, d_b, (int64_t)nb, q,
                          d_profile_a, d_profile_b);
     });
+    // every call on this GPU shares the scratch array: even on a context
+    // bound to a caller's stream, the join is finished before the lock goes
+    if (ctx->async) HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return rc;
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
     int code = f.err == hipErrorOutOfMemory ? SCAMP_HIP_OUT_OF_MEMORY
@@ -1262,7 +1328,7 @@ int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
     HIP_CHECK(launch_pearson_backward(d_T, (int64_t)n, m, batch, mean, inorm,
                                       stat_len, d_index, d_grad_corr, d_grad_T,
                                       work, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    if (!ctx->async) HIP_CHECK(hipStreamSynchronize(s));
   } catch (const HipFailure &f) {
     (void)hipStreamSynchronize(ctx->stream);
     return fail(SCAMP_HIP_DEVICE_ERROR,
@@ -1284,6 +1350,13 @@ int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
                           double *prep_ms, double *sweep_ms) {
   if (!ctx) return fail(SCAMP_HIP_INVALID_ARGS, "null context");
   std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    HIP_CHECK(hipSetDevice(ctx->device));
+    resolve_timing(ctx);
+  } catch (const HipFailure &f) {
+    return fail(SCAMP_HIP_DEVICE_ERROR,
+                std::string(f.what) + ": " + hipGetErrorString(f.err));
+  }
   if (total_ms) *total_ms = ctx->total_ms;
   if (prep_ms) *prep_ms = ctx->prep_ms;
   if (sweep_ms) *sweep_ms = ctx->sweep_ms;

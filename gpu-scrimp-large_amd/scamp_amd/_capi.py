@@ -29,7 +29,8 @@ EXPORTS = ("scamp_hip_ctx_create", "scamp_hip_ctx_destroy", "scamp_hip_device_co
            "scamp_hip_last_error", "scamp_hip_join", "scamp_hip_join_device",
            "scamp_hip_stats", "scamp_hip_qt_seed", "scamp_hip_last_timing",
            "scamp_hip_pearson_backward", "scamp_hip_selfjoin_batch",
-           "scamp_hip_last_precision", "scamp_hip_join_device_noalloc")
+           "scamp_hip_last_precision", "scamp_hip_join_device_noalloc",
+           "scamp_hip_ctx_set_stream")
 # SCAMP_HIP_NOALLOC_SCRATCH_BYTES: the no-allocation join's working memory per GPU
 NOALLOC_SCRATCH_BYTES = 128 << 20
 
@@ -78,6 +79,7 @@ def load():
     vp, dp, sz, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.c_int64
     lib.scamp_hip_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
     lib.scamp_hip_ctx_destroy.argtypes = [vp]
+    lib.scamp_hip_ctx_set_stream.argtypes = [vp, vp]
     lib.scamp_hip_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.scamp_hip_last_error.argtypes = [ctypes.c_char_p, sz]
     lib.scamp_hip_join.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(Params), vp, vp]
@@ -138,6 +140,13 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_stream(self, stream):
+        """Queue this context's work on `stream` (an integer hipStream_t
+        handle, e.g. torch.cuda.current_stream().cuda_stream; None or 0: the
+        context's own stream).  On a caller's stream the device-buffer calls
+        return once their work is queued (scamp_hip_ctx_set_stream)."""
+        check(self._lib.scamp_hip_ctx_set_stream(self._h, ctypes.c_void_p(stream or None)))
 
     def timing(self):
         t, p, s = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
@@ -265,10 +274,12 @@ def qt_seed(X, Y, m, row, diag0, count, device=0):
     return out
 
 
-def join_device(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=None, device=0):
+def join_device(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=None, device=0,
+                ctx=None):
     """Device-buffer join (scamp_hip_join_device) on raw device pointers
-    (e.g. torch tensors' data_ptr()).  Blocks until the profiles are written."""
-    ctx = context(device)
+    (e.g. torch tensors' data_ptr()).  Blocks until the profiles are written
+    (queues the join, on a context bound to a caller's stream)."""
+    ctx = ctx or context(device)
     check(ctx._lib.scamp_hip_join_device(
         ctx.handle, ctypes.c_void_p(ptr_a), na, ctypes.c_void_p(ptr_b) if ptr_b else None,
         nb, ctypes.byref(params), ctypes.c_void_p(ptr_pa),
@@ -287,12 +298,14 @@ def join_device_noalloc(ptr_a, na, ptr_b, nb, params: Params, ptr_pa, ptr_pb=Non
         ctypes.c_void_p(ptr_pb) if ptr_pb else None))
 
 
-def selfjoin_batch_device(ptr_T, n, batch, params: Params, ptr_profiles, device=0):
+def selfjoin_batch_device(ptr_T, n, batch, params: Params, ptr_profiles, device=0,
+                          ctx=None):
     """`batch` self-joins of the float64 device series at ptr_T ([batch, n],
     back to back) in one launch (scamp_hip_selfjoin_batch); the profiles at
     ptr_profiles ([batch, n - m + 1], 1NN_INDEX or 1NN) are the prior and
-    are updated in place.  Blocks until done."""
-    ctx = context(device)
+    are updated in place.  Blocks until done (queues the joins, on a context
+    bound to a caller's stream)."""
+    ctx = ctx or context(device)
     check(ctx._lib.scamp_hip_selfjoin_batch(
         ctx.handle, ctypes.c_void_p(ptr_T), n, batch, ctypes.byref(params),
         ctypes.c_void_p(ptr_profiles)))

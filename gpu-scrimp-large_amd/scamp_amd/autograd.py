@@ -13,6 +13,10 @@ kernel ``scamp_hip_pearson_backward``, which reuses the forward's indices
 (no join is recomputed) and scatters dP_i/dT into windows i and I_i:
     dr/dx = (y^ - r x^) / |x~|,   dr/dy = (x^ - r y^) / |y~|.
 The indices are piecewise constant in T and carry no gradient.
+
+Both run on torch's current stream, through an engine context of their own
+bound to it (scamp_hip_ctx_set_stream): the engine's kernels are queued
+behind torch's and torch's behind them, with no host synchronisation.
 """
 from __future__ import annotations
 
@@ -25,14 +29,29 @@ from . import _capi as C
 
 _INIT = int(C.init_profile("1NN_INDEX", 1)[0].view(np.int64))
 
+# device -> (engine context, the torch stream it is bound to)
+_BOUND = {}
+
+
+def _engine(device: int) -> C.Context:
+    """This module's engine context on `device`, bound to torch's current
+    stream there (rebound when the current stream changes)."""
+    ctx, bound = _BOUND.get(device, (None, None))
+    if ctx is None:
+        ctx = C.Context(device)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    if stream != bound:
+        ctx.set_stream(stream)
+    _BOUND[device] = (ctx, stream)
+    return ctx
+
 
 def _forward(t64: torch.Tensor, m: int, device: int):
     B, n = t64.shape
     ns = n - m + 1
     prof = torch.full((B, ns), _INIT, dtype=torch.int64, device=t64.device)
-    torch.cuda.synchronize(t64.device)
     C.selfjoin_batch_device(t64.data_ptr(), n, B, C.make_params(m, precision="double"),
-                            prof.data_ptr(), device=device)
+                            prof.data_ptr(), device=device, ctx=_engine(device))
     # mp_entry: the float32 correlation in the low word, the int32 index in
     # the high word (views, no copies)
     corr = prof.view(torch.float32).view(B, ns, 2)[..., 0]
@@ -67,9 +86,8 @@ class MatrixProfile(torch.autograd.Function):
         B, n = t64.shape
         g = torch.nan_to_num(grad_p.to(torch.float64), nan=0.0).contiguous()
         grad_t = torch.empty_like(t64)
-        torch.cuda.synchronize(t64.device)
         lib = C.load()
-        cx = C.context(ctx.device)
+        cx = _engine(ctx.device)
         C.check(lib.scamp_hip_pearson_backward(
             cx.handle, ctypes.c_void_p(t64.data_ptr()), n, ctx.m, B,
             ctypes.c_void_p(I.contiguous().data_ptr()), ctypes.c_void_p(g.data_ptr()),

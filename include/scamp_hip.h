@@ -113,6 +113,21 @@ typedef struct {
 int scamp_hip_ctx_create(int device, scamp_hip_ctx **out);
 int scamp_hip_ctx_destroy(scamp_hip_ctx *ctx);
 
+/* No counterpart in the reference (whose tiles own their streams,
+ * src/common.cpp:109-124): puts the context's work on `stream`, a hipStream_t
+ * of the context's device that the caller owns (NULL: back to the context's
+ * own stream), so that its joins order with the caller's own work on that
+ * stream (a torch stream, say) without host synchronisation.  While a
+ * caller's stream is set, scamp_hip_join_device, scamp_hip_selfjoin_batch and
+ * scamp_hip_pearson_backward return once their work is queued: their outputs
+ * are ready in stream order, and their inputs must stay valid until then.  A
+ * MIXED/SINGLE 1NN_INDEX join still waits for its FP64 audit, and the
+ * no-allocation join still finishes before it returns (it shares the GPU's
+ * scratch array); the host-buffer calls always block.
+ * scamp_hip_last_timing then waits for the last join to end.  Switching
+ * streams first waits for the work on the old one. */
+int scamp_hip_ctx_set_stream(scamp_hip_ctx *ctx, void *stream);
+
 /* Number of visible HIP devices (0 when none), as SCAMP's do_SCAMP counts
  * CUDA devices (src/SCAMP.cpp:261-275). */
 int scamp_hip_device_count(int *count);
@@ -131,7 +146,8 @@ int scamp_hip_join(scamp_hip_ctx *ctx, const double *a, size_t na,
 
 /* The same join on DEVICE buffers already resident in HBM (series as float64
  * device pointers, profiles in the layouts above).  Work is queued on the
- * context's stream; the call returns after the stream has finished. */
+ * context's stream; the call returns after the stream has finished (after
+ * queueing it, on a stream set by scamp_hip_ctx_set_stream). */
 int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
                           const double *d_b, size_t nb,
                           const scamp_hip_params *p, void *d_profile_a,
@@ -148,7 +164,8 @@ int scamp_hip_join_device(scamp_hip_ctx *ctx, const double *d_a, size_t na,
  * while the batch is small
  * enough that its bands are cut into row chunks like a single series'; up to
  * FP64 rounding of the recurrence beyond); fills the GPU when each series
- * alone has too few diagonals to.  Blocks until done. */
+ * alone has too few diagonals to.  Blocks until done (queues the work, on a
+ * stream set by scamp_hip_ctx_set_stream). */
 int scamp_hip_selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, size_t n,
                              int64_t batch, const scamp_hip_params *p,
                              void *d_profiles);
@@ -178,14 +195,15 @@ int scamp_hip_qt_seed(scamp_hip_ctx *ctx, const double *X, size_t nx,
  * with the index held fixed.  d_grad_T is overwritten; the result does not
  * depend on scheduling (no atomics: the same bits on every call).  At most
  * 2^31 - 1 profile entries (batch * (n - m + 1)) per call.  Blocks until
- * done. */
+ * done (queues the work, on a stream set by scamp_hip_ctx_set_stream). */
 int scamp_hip_pearson_backward(scamp_hip_ctx *ctx, const double *d_T, size_t n,
                                int64_t window, int64_t batch,
                                const int32_t *d_index,
                                const double *d_grad_corr, double *d_grad_T);
 
 /* Timing of the last join on this context, from HIP events on its stream:
- * total, statistics+seeds, and the diagonal sweep, in milliseconds. */
+ * total, statistics+seeds, and the diagonal sweep, in milliseconds (waits
+ * for that join to end when it was queued on a caller's stream). */
 int scamp_hip_last_timing(scamp_hip_ctx *ctx, double *total_ms,
                           double *prep_ms, double *sweep_ms);
 
