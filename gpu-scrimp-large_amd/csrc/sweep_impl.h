@@ -615,12 +615,15 @@ struct Sweep {
       ok = tra.pos + 2 * K - 2 < a.rpc && tra_col_pos + K <= a.cpc;
     return __builtin_amdgcn_ballot_w64(!ok) == 0;
   }
-  __device__ __forceinline__ void cells_fast_accumulate(const DT (&c)[K]) {
-#pragma unroll
-    for (int k = 0; k < K; k += 2) {
-      blk0 = fmax(blk0, c[k]);
-      blk1 = fmax(blk1, c[k + 1]);
-    }
+  // A fast block needs only the largest correlation of the lane's row:
+  // rounding is monotone, so for the row's inverse norm rin > 0,
+  // max_k fl(x_k * rin) = fl(max_k x_k * rin) with x_k = cov_k * in_col --
+  // one multiply by rin per row instead of one per pair, the same bits (a
+  // NaN x_k or rin is ignored by fmax either way).
+  __device__ __forceinline__ void cells_fast_accumulate(const DT (&x)[K], DT rin) {
+    const DT m0 = fmax(fmax(x[0], x[1]), fmax(x[2], x[3]));
+    const DT m1 = fmax(fmax(x[4], x[5]), fmax(x[6], x[7]));
+    blk0 = fmax(blk0, fmax(m0, m1) * rin);
   }
   __device__ __forceinline__ void cells_fast_end() {
     const double v = (double)fmax(blk0, blk1);
@@ -889,6 +892,14 @@ struct Sweep {
   __device__ __forceinline__ void row_tests(int r, int rr, int64_t i, DT rin,
                                             DT trow) {
     DT c[K];
+    if constexpr (KIND == kMatrix && FAST) {
+      static_assert(K == 8, "the max tree takes 8 correlations");
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * S[(r + k) % K].in;
+      cells_fast_accumulate(c, rin);
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const DT cv = kComp ? hi[k] + cov[k] : cov[k];
@@ -939,10 +950,7 @@ struct Sweep {
         if (ROWS) rsum[r] = rs;
       }
     } else if (KIND == kMatrix) {
-      if (FAST)
-        cells_fast_accumulate(c);
-      else
-        cells_accumulate(c);
+      cells_accumulate(c);
     } else {  // kMatrixSmall: cells narrower than a lane's 8 columns
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -1145,10 +1153,16 @@ struct Sweep {
 // spills the targets cost are cheaper than the lost wave (measured).
 // ALL_NEIGHBORS in FP32: 3 waves (12 % faster than the 2 the compiler
 // picks); in FP64 the same target spills heavily (4x slower), so none.
+// SUM_THRESH (both forms) and FP32 MATRIX_SUMMARY: 3 waves instead of the
+// compiler's 2 (measured on a 2^21 self-join, m = 1024 / 256: SUM 615 ->
+// 462 ms mixed, 612 -> 508 ms FP64; summary 526 -> 440 ms mixed); the
+// FP64 summary spills in its row loop at 3 waves (2.5x slower), so none.
 template <typename DT, int KIND>
 constexpr int sweep_waves_per_eu() {
   if (KIND == kNN) return sizeof(DT) == 4 ? 4 : 3;
   if (KIND == kKnn) return sizeof(DT) == 4 ? 3 : 1;
+  if (KIND == kSum) return 3;
+  if (KIND == kMatrix) return sizeof(DT) == 4 ? 3 : 1;
   return 1;
 }
 
