@@ -52,9 +52,10 @@ namespace sweep {
 constexpr int K = kDiagsPerLane;
 // u-samples per LDS stage of a seed / re-anchor: the FP32 form has its own
 // staging, sized (with the staged terms) to 10 KB, which lets 16 one-wave
-// workgroups share a CU's 160 KB of LDS (4 waves per SIMD); the FP64 form
-// stages through its row and column term arrays (idle at a segment start,
-// the only place it seeds), which hold 640 doubles
+// workgroups share a CU's 160 KB of LDS (the FP32 short-chunk kernel runs 4
+// waves per SIMD); the FP64 form stages through its row and column term
+// arrays (idle at a segment start, the only place it seeds), which hold 640
+// doubles
 constexpr int kAnchorChunk = 184;
 constexpr int kAnchorChunk64 = 64;
 
@@ -1147,19 +1148,22 @@ struct Sweep {
   }
 };
 
-// Occupancy targets of the 1NN kinds: 3 waves per SIMD for FP64 (<= 168
-// VGPRs) and 4 for FP32 (<= 128).  With 16-row blocks the compiler would
-// otherwise use more registers for look-ahead and drop a wave; the few
-// spills the targets cost are cheaper than the lost wave (measured).
+// Occupancy targets of the 1NN kinds: 3 waves per SIMD (<= 168 VGPRs), and
+// 4 (<= 128) for the FP32 short-chunk kernel.  The compiler would otherwise
+// use more registers for look-ahead and drop a wave; the few spills the
+// targets cost are cheaper than the lost wave (measured at n = 2^22,
+// m = 1024: FP64 1465 ms at 3 waves, 1505 at 2; FP32 two-stage 1192 ms at 3,
+// 1400 at 4, 1620 at 2; the FP32 short-chunk kernel, n = 2^17, m = 100:
+// 2.11 ms at 4, 2.18 at 3).
 // ALL_NEIGHBORS in FP32: 3 waves (12 % faster than the 2 the compiler
 // picks); in FP64 the same target spills heavily (4x slower), so none.
 // SUM_THRESH (both forms) and FP32 MATRIX_SUMMARY: 3 waves instead of the
 // compiler's 2 (measured on a 2^21 self-join, m = 1024 / 256: SUM 615 ->
 // 462 ms mixed, 612 -> 508 ms FP64; summary 526 -> 440 ms mixed); the
 // FP64 summary spills in its row loop at 3 waves (2.5x slower), so none.
-template <typename DT, int KIND>
+template <typename DT, int KIND, bool FRESH>
 constexpr int sweep_waves_per_eu() {
-  if (KIND == kNN) return sizeof(DT) == 4 ? 4 : 3;
+  if (KIND == kNN) return sizeof(DT) == 4 && FRESH ? 4 : 3;
   if (KIND == kKnn) return sizeof(DT) == 4 ? 3 : 1;
   if (KIND == kSum) return 3;
   if (KIND == kMatrix) return sizeof(DT) == 4 ? 3 : 1;
@@ -1233,7 +1237,7 @@ __device__ __forceinline__ bool plan_item(const SweepArgs &a, unsigned int idx,
 
 template <typename DT, int KIND, bool COLS, bool ROWS, bool FRESH>
 __global__ __launch_bounds__(kWave)
-__attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<DT, KIND>())))
+__attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<DT, KIND, FRESH>())))
 void sweep_kernel(SweepArgs a) {
   using SW = Sweep<DT, KIND, COLS, ROWS, FRESH>;
   __shared__ Terms<DT> s_terms[2 * kWave];  // staged row / column terms
