@@ -159,6 +159,22 @@ __device__ __forceinline__ float stage1_bound32(float tlo, float t, float in_col
   return in_col == in_col ? lowered : __builtin_inff();
 }
 
+// SUM_THRESH's first stage: the bound thr / in_col in row-scaled units,
+// lowered as stage1_bound's (a relative 2^-30 for FP64, 2^-16 for FP32,
+// and a tiny absolute amount); +inf for a column that can never match.
+__device__ __forceinline__ double sum_bound(double thr, double in_col) {
+  double r = __builtin_amdgcn_rcp(in_col);
+  r = fma(r, fma(-in_col, r, 1.0), r);
+  const double b = thr * r;
+  const double lowered = fma(-fabs(b), 0x1p-30, b) - 0x1p-1000;
+  return in_col == in_col ? lowered : __builtin_inf();
+}
+__device__ __forceinline__ float sum_bound(double thr, float in_col) {
+  const float b = (float)thr * __builtin_amdgcn_rcpf(in_col);
+  const float lowered = fmaf(-fabsf(b), 0x1p-16f, b) - 0x1p-100f;
+  return in_col == in_col ? lowered : __builtin_inff();
+}
+
 template <typename DT>
 __device__ __forceinline__ DT nan_of();
 template <>
@@ -258,7 +274,13 @@ struct Sweep {
   // that can never match scales its first stage by 0 (Terms::s1) while its
   // exact stage keeps the NaN inverse norm, and a column that can never
   // match has B = +inf.
-  uint32_t vmask = 0;  // two-stage kernels: the lane's valid diagonals
+  // SUM_THRESH tests its rows the same way, against a per-column bound
+  // thr / in_col (a constant of the column, made when it enters the band and
+  // moved with it); the exact sum runs only on rows where some pair passes.
+  static constexpr bool kSumStage = KIND == kSum;
+  // kernels whose invalid diagonals carry far_cov() (see above)
+  static constexpr bool kFarDiag = kTwoStage || kSumStage;
+  uint32_t vmask = 0;  // kFarDiag kernels: the lane's valid diagonals
   // the block's row bound: FP32 the threshold itself, FP64 its enc_thr image
   DT tlo_f = (DT)0;
   int32_t tlo_i = 0;
@@ -317,7 +339,10 @@ struct Sweep {
       else
         s.t = t;
     }
-    if (KIND == kSum) s.acc = 0.0;
+    if (KIND == kSum) {
+      s.acc = 0.0;
+      s.mt = sum_bound(a.thr, s.in);
+    }
   }
 
   // Two-stage filter: the first-stage threshold of a slot from the block's
@@ -373,6 +398,12 @@ struct Sweep {
     } else if constexpr (kTwoStage) {
       r.t = canon_thr(r.t);
       c.t = canon_thr(c.t);
+    }
+    if constexpr (kSumStage) {
+      // the entering column's bound travels in its record's t; a row that
+      // can never match scales its first stage by 0
+      c.t = sum_bound(a.thr, c.in);
+      r.s1 = r.in == r.in ? r.in : (DT)0;
     }
     if constexpr (!kTwoStage) {
       s_row[lane] = r;
@@ -469,13 +500,13 @@ struct Sweep {
     if (kTable && lane < kRows) s_rt[lane] = 0;
     if (kColTable)
       for (int q = lane; q < K * kColStride; q += kWave) s_ct[q] = 0;
-    if (kTwoStage) vmask = 0;
+    if (kFarDiag) vmask = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t d = d0 + k;
       const double v = acc[k] - (a.muX[i + d] - ref) * sy;
-      if constexpr (kTwoStage) vmask |= (diag_valid(d) ? 1u : 0u) << k;
-      if constexpr (kComp && kTwoStage) {
+      if constexpr (kFarDiag) vmask |= (diag_valid(d) ? 1u : 0u) << k;
+      if constexpr (kComp && kFarDiag) {
         const DT h = (DT)v;
         hi[k] = diag_valid(d) ? h : far_cov<DT>();
         cov[k] = diag_valid(d) ? (DT)(v - (double)h) : (DT)0;
@@ -483,7 +514,7 @@ struct Sweep {
         const DT h = (DT)v;
         hi[k] = diag_valid(d) ? h : nan_of<DT>();
         cov[k] = (DT)(v - (double)h);
-      } else if constexpr (kTwoStage) {
+      } else if constexpr (kFarDiag) {
         cov[k] = diag_valid(d) ? (DT)v : far_cov<DT>();
       } else {
         cov[k] = diag_valid(d) ? (DT)v : nan_of<DT>();
@@ -891,8 +922,32 @@ struct Sweep {
   // Row i of every other kernel: the correlations, then the kind's tests.
   template <bool MASKED, bool FAST>
   __device__ __forceinline__ void row_tests(int r, int rr, int64_t i, DT rin,
-                                            DT trow) {
+                                            DT trow, DT rs1) {
     DT c[K];
+    if constexpr (kSumStage) {
+      // first stage: some d_k = cov_k * in_row - thr / in_col >= 0
+      int32_t h[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        h[k] = sign_word(fma(kComp ? hi[k] + cov[k] : cov[k], rs1, -S[(r + k) % K].mt));
+      static_assert(K == 8, "the max tree below takes 8 differences");
+      const int32_t m0 = max(max(h[0], h[1]), h[2]);
+      const int32_t m1 = max(max(h[3], h[4]), h[5]);
+      const int32_t mx = max(max(h[6], h[7]), max(m0, m1));
+      if (__builtin_amdgcn_ballot_w64(mx >= 0) == 0) return;
+      // the exact sums, over the correlations (cov * in_col) * in_row
+      double rs = 0.0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        DT cv = (kComp ? hi[k] + cov[k] : cov[k]) * S[(r + k) % K].in * rin;
+        if (!((vmask >> k) & 1u) || (MASKED && i + d0 + k >= a.nX)) cv = nan_of<DT>();
+        const double v = cv > (DT)a.thr ? (double)cv : 0.0;
+        if (ROWS) rs += v;
+        if (COLS) S[(r + k) % K].acc += v;
+      }
+      if (ROWS) rsum[r] = rs;
+      return;
+    }
     if constexpr (KIND == kMatrix && FAST) {
       static_assert(K == 8, "the max tree takes 8 correlations");
 #pragma unroll
@@ -972,8 +1027,10 @@ struct Sweep {
     const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
     if constexpr (kTwoStage)
       row_two_stage<MASKED>(r, rr, i, rin, rv.s1, trow);
+    else if constexpr (kSumStage)
+      row_tests<MASKED, FAST>(r, rr, i, rin, trow, rv.s1);
     else
-      row_tests<MASKED, FAST>(r, rr, i, rin, trow);
+      row_tests<MASKED, FAST>(r, rr, i, rin, trow, rin);
     // advance the recurrence to row i+1
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1000,6 +1057,8 @@ struct Sweep {
     s.in = shl1(s.in, e.in);
     if constexpr (kIntCol) {
       s.F = shl1(s.F, e.F);
+      s.mt = shl1(s.mt, e.t);
+    } else if constexpr (kSumStage) {
       s.mt = shl1(s.mt, e.t);
     } else if (kThr && COLS) {
       s.t = shl1(s.t, e.t);
