@@ -152,10 +152,11 @@ def large_selfjoin(args):
     """Self-join of a float32 random walk, n = 2^25 (PCG64 seed 5; statistics
     in FP64 from the float32 values), m = 4096: the mixed form's 1NN_INDEX
     profile and a 1024 x 1024 MATRIX_SUMMARY, each timed once, and the FP64
-    1NN_INDEX profile as the error reference.  Checks: every row of the
-    summary peaks at the maximum of the 1NN profile over that row band (same
-    FP32 arithmetic, so exactly); the FP64 profile at sampled rows equals
-    the exact FP64 correlation of the reported pair."""
+    1NN_INDEX profile as the error reference.  Checks: for every band r of
+    subsequences, the larger of the summary's row r and column r maxima (the
+    summary holds each pair i < j once, in cell (band of i, band of j)) is
+    the maximum of the 1NN profile over that band; the FP64 profile at
+    sampled rows equals the exact FP64 correlation of the reported pair."""
     import torch
     from scamp_amd import _capi as C
     from scamp_amd import tuning
@@ -195,7 +196,9 @@ def large_selfjoin(args):
     valid = c64 > -1
     dcorr = float(np.max(np.abs(c32[valid].astype(np.float64) - c64[valid])))
     agree = float(np.mean(i32[valid] == i64[valid]))
-    # summary rows against the profile: cell row r covers rows [r*rpc, (r+1)*rpc)
+    # summary bands against the profile: band r covers subsequences
+    # [r*rpc, (r+1)*rpc); the pairs of band r sit in cell row r (as the
+    # earlier subsequence) and cell column r (as the later one)
     rpc = -(-ns // H)
     prof_max = np.full(H, -2.0, dtype=np.float32)
     for r in range(H):
@@ -203,7 +206,8 @@ def large_selfjoin(args):
         if seg.size:
             prof_max[r] = seg.max()
     prof_max = np.where(prof_max > 0.0, prof_max, np.float32(-2.0))  # cells keep values > 0
-    row_max = np.max(np.where(np.isnan(msum), -2.0, msum), axis=1)
+    filled = np.where(np.isnan(msum), -2.0, msum)
+    row_max = np.maximum(filled.max(axis=1), filled.max(axis=0))
     # the summary keeps the sweep's FP32 values, the 1NN_INDEX profile its
     # FP64-refined ones: they agree to the mixed form's rounding
     summary_diff = float(np.max(np.abs(row_max.astype(np.float64) - prof_max)))

@@ -161,8 +161,6 @@ struct SweepArgs {
   double *sumX, *sumY;   // SUM_THRESH profiles
   uint32_t *mat;         // MATRIX_SUMMARY cells (order-preserving uint32)
   int64_t mat_w, rpc, cpc;
-  int32_t mat_mirror;    // rows and columns on one cell grid (self-join,
-                         // rpc == cpc): a transposed cell mirrors a direct one
   // ALL_NEIGHBORS: per subsequence a list of knn_k order-preserving keys (the
   // k best matches so far; keyX / keyY then hold each list's minimum)
   uint64_t *knnX, *knnY;

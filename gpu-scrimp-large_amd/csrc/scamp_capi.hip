@@ -453,7 +453,6 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
   a.keyX = p.keyX; a.keyY = p.keyY; a.sumX = p.sumX; a.sumY = p.sumY;
   a.knnX = p.knnX; a.knnY = p.knnY; a.knn_k = knn_k;
   a.mat = mat; a.mat_w = mat_w; a.rpc = rpc; a.cpc = cpc; a.thr = thr;
-  a.mat_mirror = p.X == p.Y && rpc == cpc;
   a.counter = reinterpret_cast<unsigned int *>(counter);
   a.nbatch = nbatch;
   a.batch_stride = stride;
@@ -708,7 +707,14 @@ int join_device(scamp_hip_ctx *ctx, Arena &ar, const double *d_a, int64_t na,
       Pass ps{&A, &A, r.lo[i], r.hi[i], cols_out, rows_out, keyA,
               sep_rows ? keyB : keyA, sumA, sep_rows ? sumB : sumA,
               knnA, sep_rows ? knnB : knnA};
-      if (pt == SCAMP_HIP_MATRIX_SUMMARY) ps.cols = ps.rows = true;
+      // MATRIX_SUMMARY of a self-join: the cells of the pairs (row i, column
+      // j > i), on and above the diagonal, as the reference's
+      // upper-triangular tiles give them with rows switched off
+      // (src/SCAMP.cpp:34-43, src/common.cpp:98-106); cells below stay empty
+      if (pt == SCAMP_HIP_MATRIX_SUMMARY) {
+        ps.cols = true;
+        ps.rows = false;
+      }
       passes[npass++] = ps;
     }
   } else {

@@ -300,15 +300,13 @@ struct Sweep {
   double rsum[K];
   Slot<DT> S[K];         // column slots, indexed by column mod K
   // MATRIX_SUMMARY state
-  const bool mirror;     // transposed cells mirror direct ones: tra unused
   DT blk0, blk1;         // fast blocks: running max of the lane's pairs
   CellAcc dir, tra;
   int64_t dir_row_cell, dir_row_pos;  // uniform: row cell and row mod rpc
   int64_t tra_col_cell, tra_col_pos;  // uniform: column cell and row mod cpc
 
   __device__ Sweep(const SweepArgs &args, int ln, int64_t band_d0)
-      : a(args), lane(ln), D0(band_d0), d0(band_d0 + (int64_t)ln * K),
-        mirror(COLS && ROWS && args.mat_mirror) {}
+      : a(args), lane(ln), D0(band_d0), d0(band_d0 + (int64_t)ln * K) {}
 
   __device__ __forceinline__ bool diag_valid(int64_t d) const {
     return d >= a.d_lo && d < a.d_hi;
@@ -539,7 +537,6 @@ struct Sweep {
     if (v > a.thr) {
       const uint32_t e = enc_corr((float)v);
       atomicMax(&a.mat[r * a.mat_w + c], e);
-      if (mirror) atomicMax(&a.mat[c * a.mat_w + r], e);
     }
   }
   __device__ __forceinline__ void cells_begin(int64_t i) {
@@ -551,7 +548,7 @@ struct Sweep {
       dir_row_cell = i / a.rpc;
       dir_row_pos = i - dir_row_cell * a.rpc;
     }
-    if (ROWS && !mirror) {  // transposed cells (column / rpc, row i / cpc)
+    if (ROWS) {  // transposed cells (column / rpc, row i / cpc)
       tra.lo = tra.hi = nan_of<double>();
       tra.lo_cell = col0 / a.rpc;
       tra.pos = col0 - tra.lo_cell * a.rpc;
@@ -565,7 +562,7 @@ struct Sweep {
       cell_flush(dir.hi, dir_row_cell, dir.lo_cell + 1);
       dir.lo = dir.hi = nan_of<double>();
     }
-    if (ROWS && !mirror) {
+    if (ROWS) {
       cell_flush(tra.lo, tra.lo_cell, tra_col_cell);
       cell_flush(tra.hi, tra.lo_cell + 1, tra_col_cell);
       tra.lo = tra.hi = nan_of<double>();
@@ -579,7 +576,7 @@ struct Sweep {
       dir.lo = dir.hi = nan_of<double>();
       dir_row_cell += 1;
     }
-    if (ROWS && !mirror && tra_col_pos == 0 && !first) {
+    if (ROWS && tra_col_pos == 0 && !first) {
       cell_flush(tra.lo, tra.lo_cell, tra_col_cell);
       cell_flush(tra.hi, tra.lo_cell + 1, tra_col_cell);
       tra.lo = tra.hi = nan_of<double>();
@@ -597,7 +594,7 @@ struct Sweep {
         dir.hi = fmax(dir.hi, in_lo ? nan_of<double>() : v);
       }
     }
-    if (ROWS && !mirror) {
+    if (ROWS) {
       const int64_t kb = a.rpc - tra.pos;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -620,7 +617,7 @@ struct Sweep {
       }
       if (++dir_row_pos == a.rpc) dir_row_pos = 0;
     }
-    if (ROWS && !mirror) {
+    if (ROWS) {
       if (++tra.pos == a.rpc) {
         cell_flush(tra.lo, tra.lo_cell, tra_col_cell);
         tra.lo = tra.hi;
@@ -639,7 +636,7 @@ struct Sweep {
   // Wave-uniform decision; blocks that touch a boundary take the per-pair
   // path above.
   __device__ __forceinline__ bool cells_block_fast() const {
-    if (COLS && ROWS && !mirror) return false;
+    if (COLS && ROWS) return false;
     bool ok;
     if (COLS)
       ok = dir.pos + 2 * K - 2 < a.cpc && dir_row_pos + K <= a.rpc;

@@ -77,6 +77,14 @@ def reduce_sum(D, thr):
     return np.where(D > thr, D, 0.0).sum(axis=0)
 
 
+def upper_pairs(D):
+    """A self-join distance matrix with only the pairs (row i, column j > i):
+    what a self-join MATRIX_SUMMARY sees (the reference's upper-triangular
+    tiles with rows switched off, src/SCAMP.cpp:34-43, src/common.cpp:98-106)."""
+    ii, jj = np.indices(D.shape)
+    return np.where(jj > ii, D, -2.0)
+
+
 def reduce_matrix(D, H, W, thr):
     rpc = -(-D.shape[0] // H)
     cpc = -(-D.shape[1] // W)

@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import corr_matrix, random_walk, reduce_1nn_index, reduce_matrix, reduce_sum
+from helpers import corr_matrix, random_walk, reduce_1nn_index, reduce_matrix, reduce_sum, upper_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -121,7 +121,7 @@ def test_other_profile_types_and_pearson_output(tmp_path):
     lines = (tmp_path / "summary").read_text().strip("\n").split("\n")
     assert len(lines) == 5 and all(len(l.split()) == 7 for l in lines)
     got = np.array([[float(x) for x in l.split()] for l in lines])
-    ref = reduce_matrix(D, 5, 7, 0.0)
+    ref = reduce_matrix(upper_pairs(D), 5, 7, 0.0)
     ref = np.where(ref <= -2, np.nan, ref)
     assert np.array_equal(np.isnan(got), np.isnan(ref))
     assert np.nanmax(np.abs(got - ref)) <= 1e-6

@@ -13,9 +13,11 @@ FP64 dot products; nothing here uses the engine's own FP64 form):
     of the centred window against the series, the best candidates recomputed
     directly): the stored value is within 1e-5 of the row's true maximum, and
     an index that differs from the true argmax is a near-tie within 1e-5;
-  * every row of the summary peaks at the profile's maximum over that row
-    band (src/common.cpp:98-106 cell geometry: rows per cell =
-    ceil(n_sub / 1024)), up to the mixed form's rounding.
+  * for every band of subsequences, the larger of the summary's row and
+    column maxima for that band (a self-join summary holds each pair i < j
+    once, in cell (band of i, band of j); src/common.cpp:98-106 cell
+    geometry: rows per cell = ceil(n_sub / 1024)) is the profile's maximum
+    over the band, up to the mixed form's rounding.
 The run takes a few minutes; progress is printed as it goes.
 """
 import time
@@ -133,10 +135,13 @@ def test_large_selfjoin_mixed_profile_and_summary(capsys):
             assert exact_corr(a, a, M, np.array([r]), np.array([i[r]]))[0] >= best - BAR
     log(f"{FFT_ROWS} full FP64 rows: max |stored - true row max| {worst_gap:.3g}", t0)
 
-    # summary rows against the profile's band maxima
+    # summary bands against the profile's band maxima: the pairs of band r
+    # sit in cell row r (as the earlier subsequence) and cell column r (as
+    # the later one)
     rpc = -(-ns // H)
     band_max = np.array([c[r * rpc:(r + 1) * rpc].max() for r in range(H)], dtype=np.float64)
-    row_max = np.nanmax(np.where(np.isnan(summary), -2.0, summary), axis=1).astype(np.float64)
+    filled = np.where(np.isnan(summary), -2.0, summary)
+    row_max = np.maximum(filled.max(axis=1), filled.max(axis=0)).astype(np.float64)
     diff = float(np.max(np.abs(row_max - np.where(band_max > 0, band_max, -2.0))))
     log(f"summary row maxima vs profile band maxima: max |diff| {diff:.3g}", t0)
     assert diff <= 2e-6

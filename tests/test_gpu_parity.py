@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from helpers import (check_1nn_index, corr_matrix, euclidean, exact_corr, golden,
-                     random_walk, reduce_matrix, reduce_sum)
+                     random_walk, reduce_matrix, reduce_sum, upper_pairs)
 from oracle import oracle as O
 from scamp_amd import _capi as C
 
@@ -208,9 +208,9 @@ def test_sum_thresh_matches_oracle_and_brute_force(thr):
                                              (400, None, 20, 50, 50, 0.0),
                                              (3000, 2000, 64, 10, 5, 0.125),
                                              # wide cells: whole blocks of rows
-                                             # fall in one cell (square self-join
-                                             # grid: mirrored cells; AB: each
-                                             # pass on its own side)
+                                             # fall in one cell (self-join: the
+                                             # cells on and above the diagonal;
+                                             # AB: each pass on its own side)
                                              (6000, None, 64, 10, 10, 0.0),
                                              (9000, None, 100, 7, 7, 0.3),
                                              (5000, 7000, 50, 6, 4, 0.1)])
@@ -218,7 +218,7 @@ def test_matrix_summary_is_exact_max_pool(na, nb, m, H, W, thr):
     a = random_walk(na, na + 3)
     b = None if nb is None else random_walk(nb, nb + 4)
     D = corr_matrix(a, b, m, b is None)
-    ref = reduce_matrix(D, H, W, thr)
+    ref = reduce_matrix(D if b is not None else upper_pairs(D), H, W, thr)
     pm, _ = C.join(a, b, C.make_params(m, "MATRIX_SUMMARY", has_b=b is not None,
                                        threshold=thr, matrix_height=H, matrix_width=W))
     pm = pm.reshape(H, W)

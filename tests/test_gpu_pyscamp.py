@@ -7,7 +7,7 @@ matrix, plus the Euclidean/Pearson output conversions
 import numpy as np
 import pytest
 
-from helpers import corr_matrix, reduce_1nn_index, reduce_matrix, reduce_sum
+from helpers import corr_matrix, reduce_1nn_index, reduce_matrix, reduce_sum, upper_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -97,7 +97,7 @@ def test_matrix_summary(data):
     assert np.allclose(ref, m, equal_nan=True, atol=2 ** -23)
     rw = np.cumsum(np.random.RandomState(4).standard_normal(4000))
     ms = mp.selfjoin_matrix(rw, 100, pearson=True)  # default 50 x 50
-    ref = reduce_matrix(corr_matrix(rw, None, 100, True), 50, 50, 0.0)
+    ref = reduce_matrix(upper_pairs(corr_matrix(rw, None, 100, True)), 50, 50, 0.0)
     ref = np.where(ref <= -2, np.nan, ref)
     assert np.allclose(ref, ms, equal_nan=True, atol=2 ** -23)
 
