@@ -69,6 +69,10 @@ struct Slot {
   // instead of t.
   DT mt;
   int32_t F;
+  // FP64 two-stage filter: the column's first-stage factor (stage1_factor),
+  // moved with the column in place of in, which only the exact stage reads
+  // (from the inverse-norm array)
+  DT rl;
   double acc;      // SUM: partial sum of the column, travelling with it
 };
 
@@ -87,7 +91,8 @@ struct Terms<double> {
   double t;
   int32_t F;  // two-stage filter: enc_thr of the column's threshold
   double s1;  // two-stage filter, rows: the first stage's scale (in, or 0
-              // for a row that can never match)
+              // for a row that can never match); columns: the first-stage
+              // factor (stage1_factor)
 };
 
 // Integer image of an FP64 threshold t for the two-stage filter: for c >= 0,
@@ -120,18 +125,26 @@ __device__ __forceinline__ double mt_of(int32_t tlo, int32_t f) {
   return __hiloint2double(tlo < f ? tlo : f, 0);
 }
 // The FP64 first stage in row-scaled units: a pair passes when the exact
-// product cov * in_row is at least B, B = mt_of(tlo, f) / in_col less a
-// relative 2^-30 and an absolute 2^-1000, which (s * in_col >= threshold
-// implies s >= B under FP64 rounding, with the reciprocal good to far better
-// than 2^-31 after one Newton step) keeps the stage conservative.  The test
-// is the sign of fma(cov, in_row, -B), so B is never NaN: a column that can
-// never match (in_col = NaN) gets B = +inf, which no pair reaches.
-__device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, double in_col) {
-  double r = __builtin_amdgcn_rcp(in_col);
-  r = fma(r, fma(-in_col, r, 1.0), r);
-  const double b = mt_of(tlo, f) * r;
-  const double lowered = fma(-fabs(b), 0x1p-30, b) - 0x1p-1000;
-  return in_col == in_col ? lowered : __builtin_inf();
+// product s = cov * in_row is at least B = fl(mt_of(tlo, f) * q - 2^-1000),
+// q = stage1_factor(in_col) (a column constant that travels with the column,
+// so that a block's refresh of B is one FMA).  q is one Newton step from the
+// hardware reciprocal r = (1 + e) / in_col with 2^-30 taken off inside the
+// step: q = r (2 - in_col r) - 2^-30 r = (1 - e^2 - 2^-30 (1 + e)) / in_col
+// up to two roundings of 2^-53, below (1 - 2^-31) / in_col whatever the
+// reciprocal's error e (|e| < 1/4).  So for a threshold image m >= 0,
+// B < m / in_col by a relative 2^-31 and an absolute 2^-1000, while
+// s * in_col >= threshold >= m gives s >= m / in_col less a relative 2^-52:
+// the stage is conservative.  The negative image (-1.0625) gives B below
+// -1.0625 (1 - 2^-30) / in_col, under every correlation >= -1.  A column that
+// can never match (in_col NaN) has q = +inf and B = +inf, -inf or NaN; its
+// pairs may pass the first stage but never the exact one.
+__device__ __forceinline__ double stage1_factor(double in_col) {
+  const double r = __builtin_amdgcn_rcp(in_col);
+  const double q = fma(r, fma(-in_col, r, 1.0 - 0x1p-30), r);
+  return in_col == in_col ? q : __builtin_inf();
+}
+__device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, double q) {
+  return fma(mt_of(tlo, f), q, -0x1p-1000);
 }
 // Two-stage kernels: the covariance an invalid diagonal (outside the pass,
 // at a band's edge) carries instead of NaN, so that its first-stage
@@ -152,7 +165,10 @@ __device__ __forceinline__ int32_t sign_word(float x) { return __float_as_int(x)
 // The FP32 first stage in row-scaled units (as stage1_bound): B =
 // min(tlo, t) / in_col less a relative 2^-16 (far above the FP32 rounding of
 // the product and of the hardware reciprocal) and an absolute 2^-100; +inf
-// for a column that can never match.
+// for a column that can never match.  (Carrying a lowered reciprocal with
+// the column as the FP64 form does, with the exact stage reading in_col from
+// memory, measured slower here: 1193 -> 1284 ms at n = 2^22, m = 1024; the
+// FP32 form's 64-row blocks spend half as much per row on the refresh.)
 __device__ __forceinline__ float stage1_bound32(float tlo, float t, float in_col) {
   const float b = min_thr(tlo, t) * __builtin_amdgcn_rcpf(in_col);
   const float lowered = fmaf(-fabsf(b), 0x1p-16f, b) - 0x1p-100f;
@@ -269,11 +285,12 @@ struct Sweep {
   // or bit pattern (FP32) of the d_k is >= 0 as an integer, i.e. some
   // d_k >= 0: 8 FMAs and a max3 tree per row in place of 8 multiplies and 8
   // compares, each of which would hand a mask to the scalar unit.  No d_k
-  // may be NaN (its sign bit is not defined): invalid diagonals carry
-  // far_cov() instead of NaN (the exact stage masks them with vmask), a row
-  // that can never match scales its first stage by 0 (Terms::s1) while its
-  // exact stage keeps the NaN inverse norm, and a column that can never
-  // match has B = +inf.
+  // of a pair that can match may be NaN (its sign bit is not defined):
+  // invalid diagonals carry far_cov() instead of NaN (the exact stage masks
+  // them with vmask), a row that can never match scales its first stage by 0
+  // (Terms::s1) while its exact stage keeps the NaN inverse norm; a column
+  // that can never match may pass the first stage (its bound is +-inf or
+  // NaN) but not the exact one.
   // SUM_THRESH tests its rows the same way, against a per-column bound
   // thr / in_col (a constant of the column, made when it enters the band and
   // moved with it); the exact sum runs only on rows where some pair passes.
@@ -327,6 +344,7 @@ struct Sweep {
     s.df = ld<DT>(a.dfX, off);
     s.dg = ld<DT>(a.dgX, off);
     s.in = ld<DT>(a.inX, off);
+    if constexpr (kIntCol) s.rl = stage1_factor(s.in);
     if (kThr && COLS) {
       const DT t = Thresh<DT>::from_key_hi(
           ld<uint32_t>(a.keyX, static_cast<uint32_t>(col) * 8u + 4u));
@@ -347,7 +365,7 @@ struct Sweep {
   // row bound and the column's threshold.
   __device__ __forceinline__ void refresh_mt(Slot<DT> &s) const {
     if constexpr (kIntCol)
-      s.mt = stage1_bound(tlo_i, s.F, s.in);
+      s.mt = stage1_bound(tlo_i, s.F, s.rl);
     else
       s.mt = stage1_bound32(tlo_f, s.t, s.in);
   }
@@ -420,7 +438,8 @@ struct Sweep {
         }
         tlo_i = __builtin_amdgcn_readfirstlane(v);
         // the entering column's first-stage bound travels in its record's t
-        c.t = stage1_bound(tlo_i, c.F, c.in);
+        c.s1 = stage1_factor(c.in);
+        c.t = stage1_bound(tlo_i, c.F, c.s1);
         r.s1 = r.in == r.in ? r.in : 0.0;
         s_row[lane] = r;
         s_col[lane] = c;
@@ -900,12 +919,19 @@ struct Sweep {
     const int32_t mx = max(max(h[6], h[7]), max(m0, m1));
     if (__builtin_amdgcn_ballot_w64(mx >= 0) == 0) return;
     // exact stage: the correlations (cov * in_row) * in_col, NaN where the
-    // diagonal is outside the pass or the column past the series
+    // diagonal is outside the pass or the column past the series.  FP64: the
+    // columns' inverse norms come from memory (rows get here rarely; the
+    // columns i + d0 + k lie below nX + kBand, inside the padded array).
+    DT inc[K];
+    const uint32_t coff = static_cast<uint32_t>(i + d0) * sizeof(DT);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      inc[k] = kIntCol ? ld<DT>(a.inX, coff + k * (uint32_t)sizeof(DT)) : S[(r + k) % K].in;
     DT c[K];
     uint64_t hit = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * rin * S[(r + k) % K].in;
+      c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * rin * inc[k];
       if (!((vmask >> k) & 1u) || (MASKED && i + d0 + k >= a.nX)) c[k] = nan_of<DT>();
       hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
       hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
@@ -1051,7 +1077,10 @@ struct Sweep {
     }
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
-    s.in = shl1(s.in, e.in);
+    if constexpr (kIntCol)
+      s.rl = shl1(s.rl, e.s1);
+    else
+      s.in = shl1(s.in, e.in);
     if constexpr (kIntCol) {
       s.F = shl1(s.F, e.F);
       s.mt = shl1(s.mt, e.t);
