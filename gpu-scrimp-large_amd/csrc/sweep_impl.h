@@ -72,7 +72,7 @@ struct Slot {
   // FP64 two-stage filter: the column's first-stage factor (stage1_factor),
   // moved with the column in place of in, which only the exact stage reads
   // (from the inverse-norm array)
-  DT rl;
+  float rq;
   double acc;      // SUM: partial sum of the column, travelling with it
 };
 
@@ -90,9 +90,9 @@ struct Terms<double> {
   double df, dg, in;
   double t;
   int32_t F;  // two-stage filter: enc_thr of the column's threshold
+  float q;    // two-stage filter, columns: the first-stage factor
   double s1;  // two-stage filter, rows: the first stage's scale (in, or 0
-              // for a row that can never match); columns: the first-stage
-              // factor (stage1_factor)
+              // for a row that can never match)
 };
 
 // Integer image of an FP64 threshold t for the two-stage filter: for c >= 0,
@@ -129,22 +129,26 @@ __device__ __forceinline__ double mt_of(int32_t tlo, int32_t f) {
 // q = stage1_factor(in_col) (a column constant that travels with the column,
 // so that a block's refresh of B is one FMA).  q is one Newton step from the
 // hardware reciprocal r = (1 + e) / in_col with 2^-30 taken off inside the
-// step: q = r (2 - in_col r) - 2^-30 r = (1 - e^2 - 2^-30 (1 + e)) / in_col
-// up to two roundings of 2^-53, below (1 - 2^-31) / in_col whatever the
-// reciprocal's error e (|e| < 1/4).  So for a threshold image m >= 0,
-// B < m / in_col by a relative 2^-31 and an absolute 2^-1000, while
+// step: r (2 - in_col r) - 2^-30 r = (1 - e^2 - 2^-30 (1 + e)) / in_col up to
+// two roundings of 2^-53, below (1 - 2^-31) / in_col whatever the
+// reciprocal's error e (|e| < 1/4), then rounded down to FP32 (FLT_MAX at
+// most; one DPP move per row instead of two).  So for a threshold image
+// m >= 0, B < m / in_col by a relative 2^-31 and an absolute 2^-1000, while
 // s * in_col >= threshold >= m gives s >= m / in_col less a relative 2^-52:
 // the stage is conservative.  The negative image (-1.0625) gives B below
-// -1.0625 (1 - 2^-30) / in_col, under every correlation >= -1.  A column that
-// can never match (in_col NaN) has q = +inf and B = +inf, -inf or NaN; its
-// pairs may pass the first stage but never the exact one.
-__device__ __forceinline__ double stage1_factor(double in_col) {
+// -1.0625 (1 - 2^-30) / in_col, under every correlation >= -1.  A column
+// that can never match (in_col NaN) has q = +inf and B = +inf, -inf or NaN;
+// its pairs may pass the first stage but never the exact one.
+__device__ __forceinline__ float stage1_factor(double in_col) {
   const double r = __builtin_amdgcn_rcp(in_col);
   const double q = fma(r, fma(-in_col, r, 1.0 - 0x1p-30), r);
-  return in_col == in_col ? q : __builtin_inf();
+  if (!(in_col == in_col)) return __builtin_inff();
+  float f = (float)q;  // nearest; one step down where that rounded up
+  if ((double)f > q) f = __int_as_float(__float_as_int(f) - 1);
+  return f;
 }
-__device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, double q) {
-  return fma(mt_of(tlo, f), q, -0x1p-1000);
+__device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, float q) {
+  return fma(mt_of(tlo, f), (double)q, -0x1p-1000);
 }
 // Two-stage kernels: the covariance an invalid diagonal (outside the pass,
 // at a band's edge) carries instead of NaN, so that its first-stage
@@ -344,7 +348,7 @@ struct Sweep {
     s.df = ld<DT>(a.dfX, off);
     s.dg = ld<DT>(a.dgX, off);
     s.in = ld<DT>(a.inX, off);
-    if constexpr (kIntCol) s.rl = stage1_factor(s.in);
+    if constexpr (kIntCol) s.rq = stage1_factor(s.in);
     if (kThr && COLS) {
       const DT t = Thresh<DT>::from_key_hi(
           ld<uint32_t>(a.keyX, static_cast<uint32_t>(col) * 8u + 4u));
@@ -365,7 +369,7 @@ struct Sweep {
   // row bound and the column's threshold.
   __device__ __forceinline__ void refresh_mt(Slot<DT> &s) const {
     if constexpr (kIntCol)
-      s.mt = stage1_bound(tlo_i, s.F, s.rl);
+      s.mt = stage1_bound(tlo_i, s.F, s.rq);
     else
       s.mt = stage1_bound32(tlo_f, s.t, s.in);
   }
@@ -438,8 +442,8 @@ struct Sweep {
         }
         tlo_i = __builtin_amdgcn_readfirstlane(v);
         // the entering column's first-stage bound travels in its record's t
-        c.s1 = stage1_factor(c.in);
-        c.t = stage1_bound(tlo_i, c.F, c.s1);
+        c.q = stage1_factor(c.in);
+        c.t = stage1_bound(tlo_i, c.F, c.q);
         r.s1 = r.in == r.in ? r.in : 0.0;
         s_row[lane] = r;
         s_col[lane] = c;
@@ -1078,7 +1082,7 @@ struct Sweep {
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
     if constexpr (kIntCol)
-      s.rl = shl1(s.rl, e.s1);
+      s.rq = shl1(s.rq, e.q);
     else
       s.in = shl1(s.in, e.in);
     if constexpr (kIntCol) {
