@@ -434,13 +434,7 @@ struct Sweep {
       // them; rows that can never match do not count)
       const bool mine = lane < kRows && r.in == r.in;
       if constexpr (kIntCol) {
-        int32_t v = mine ? enc_thr(r.t) : kNanEnc;
-#pragma unroll
-        for (int off = kRows / 2; off > 0; off >>= 1) {
-          const int32_t o = __shfl_xor(v, off);
-          v = o < v ? o : v;
-        }
-        tlo_i = __builtin_amdgcn_readfirstlane(v);
+        tlo_i = rows_min(mine ? enc_thr(r.t) : kNanEnc);
         // the entering column's first-stage bound travels in its record's t
         c.q = stage1_factor(c.in);
         c.t = stage1_bound(tlo_i, c.F, c.q);
@@ -448,10 +442,7 @@ struct Sweep {
         s_row[lane] = r;
         s_col[lane] = c;
       } else {
-        float v = mine ? (float)r.t : __builtin_inff();
-#pragma unroll
-        for (int off = kRows / 2; off > 0; off >>= 1) v = min_thr(v, __shfl_xor(v, off));
-        tlo_f = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+        tlo_f = __int_as_float(rows_min(__float_as_int(mine ? (float)r.t : __builtin_inff())));
         c.b = stage1_bound32(tlo_f, c.t, c.in);
         r.s1 = r.in == r.in ? r.in : 0.0f;
         s_row[lane] = r;
@@ -459,6 +450,24 @@ struct Sweep {
       }
     }
     __builtin_amdgcn_wave_barrier();
+  }
+
+  // The smallest v of lanes [0, kRows) (kRows a multiple of 16), uniform:
+  // DPP rotations within each row of 16 lanes (row_ror 8, 4, 2, 1), then one
+  // lane read per row; no LDS round trips.
+  __device__ __forceinline__ static int32_t rows_min(int32_t v) {
+    static_assert(kRows % 16 == 0, "whole DPP rows");
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x122, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x121, 0xf, 0xf, false));
+    int32_t t = __builtin_amdgcn_readlane(v, 0);
+#pragma unroll
+    for (int row = 1; row < kRows / 16; ++row) {
+      const int32_t u = __builtin_amdgcn_readlane(v, 16 * row);
+      t = u < t ? u : t;
+    }
+    return t;
   }
 
   // ---------------------------------------------------------------- seeds
