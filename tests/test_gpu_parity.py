@@ -140,6 +140,26 @@ def test_selfjoin_is_deterministic():
     assert np.array_equal(p1, p2)  # order-independent merges: bit-identical
 
 
+@pytest.mark.parametrize("scale", [2.0 ** 140, 2.0 ** -10])
+def test_join_is_scale_invariant(scale):
+    """A power-of-two scale of the series changes no rounding in the
+    statistics, seeds, recurrence or correlations (nothing overflows or
+    underflows here), so the profiles are bit-identical.  At 2^140 the inverse
+    norms fall below 2^-128 and the first-stage factor (a lowered reciprocal
+    of in_col, carried in FP32) saturates at FLT_MAX: the bound turns loose but
+    stays conservative, which this pins for the self-join and for an AB-join
+    with both profiles (both run the two-stage filter)."""
+    a = random_walk(20000, 12, 3)
+    b = random_walk(9000, 13, 2)
+    p1, _ = C.join(a, None, C.make_params(256))
+    p2, _ = C.join(a * scale, None, C.make_params(256))
+    assert np.array_equal(p1, p2)
+    params = C.make_params(100, has_b=True, keep_rows_separate=True)
+    qa, qb = C.join(a, b, params)
+    ra, rb = C.join(a * scale, b * scale, params)
+    assert np.array_equal(qa, ra) and np.array_equal(qb, rb)
+
+
 # ---------------------------------------------------- golden files
 
 
