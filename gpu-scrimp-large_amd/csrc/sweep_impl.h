@@ -277,11 +277,13 @@ struct Sweep {
   // exact row and column tests.  The first stage works in row-scaled units:
   // s = cov * in_row against mt = min(T_lo, column threshold) / in_col
   // (a per-column bound, made when the column enters the band or a block
-  // starts, that travels with the column), so a pair costs one multiply and
-  // one compare; the exact correlation s * in_col is formed only on rows that
-  // pass.  With converged thresholds a few percent of rows do.  The FP64
-  // form keeps each column's threshold as an integer image (one register,
-  // one DPP move) for its exact column test.
+  // starts, that travels with the column), so a pair costs one FMA and a
+  // share of a max tree (below); the exact correlation s * in_col is formed
+  // only on rows that pass.  With converged thresholds a few percent of rows
+  // do.  The FP64 form keeps each column's threshold as an integer image
+  // (one register, one DPP move) for its exact column test, and carries the
+  // column's first-stage factor (stage1_factor) in place of in_col, which
+  // its exact stage reads from memory.
   static constexpr bool kTwoStage = KIND == kNN && !FRESH && COLS && ROWS;
   static constexpr bool kIntCol = kTwoStage && sizeof(DT) == 8;
   // The first stage is tested by sign: d_k = fma(cov_k, in_row, -B_k) for
@@ -293,8 +295,8 @@ struct Sweep {
   // invalid diagonals carry far_cov() instead of NaN (the exact stage masks
   // them with vmask), a row that can never match scales its first stage by 0
   // (Terms::s1) while its exact stage keeps the NaN inverse norm; a column
-  // that can never match may pass the first stage (its bound is +-inf or
-  // NaN) but not the exact one.
+  // that can never match has the bound +inf (FP32), or +-inf or NaN (FP64:
+  // such a column may pass the first stage, never the exact one).
   // SUM_THRESH tests its rows the same way, against a per-column bound
   // thr / in_col (a constant of the column, made when it enters the band and
   // moved with it); the exact sum runs only on rows where some pair passes.
