@@ -13,7 +13,11 @@
  *                  high 32 bits = index; "no match" = {-2.0f, 0xFFFFFFFF}
  *   1NN            float correlation, "no match" = -2
  *   SUM_THRESH     double sum of correlations above the threshold, init 0
- *   MATRIX_SUMMARY float[height * width] row-major, init -2
+ *   MATRIX_SUMMARY float[height * width] row-major, init -2; cell (r, c)
+ *                  holds the largest correlation of the pairs (row window
+ *                  in band r, column window in band c); a self-join fills
+ *                  the pairs i < j only (cells below the diagonal keep
+ *                  their prior), as SCAMP's upper-triangular tiles do
  *   ALL_NEIGHBORS  scamp_hip_match[n_sub * k]: for subsequence j its k best
  *                  matches, entries j*k .. j*k+k-1, best first; an empty
  *                  entry has corr -2 and row 0xFFFFFFFF (k =
