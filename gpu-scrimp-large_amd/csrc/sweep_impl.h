@@ -739,9 +739,10 @@ struct Sweep {
     // NaN of an invalid diagonal passes it and is caught here).  The high
     // word of a negative double orders the other way round as an integer,
     // so a column whose threshold is negative (kNegEnc) takes every
-    // correlation, negative ones included.
+    // correlation, negative ones included.  Non-short-circuit operators: no
+    // branch (an exec-mask change) per test.
     if constexpr (kIntCol)
-      return c == c && (__double2hiint(c) >= s.F || s.F == kNegEnc);
+      return (c == c) & ((__double2hiint(c) >= s.F) | (s.F == kNegEnc));
     else
       return c >= s.t;
   }
@@ -948,7 +949,16 @@ struct Sweep {
     for (int k = 0; k < K; ++k) {
       c[k] = (kComp ? hi[k] + cov[k] : cov[k]) * rin * inc[k];
       if (!((vmask >> k) & 1u) || (MASKED && i + d0 + k >= a.nX)) c[k] = nan_of<DT>();
-      hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
+      if constexpr (kIntCol) {
+        // col_passes as the masks of single compares (no round trip of a
+        // boolean through a register)
+        const int32_t F = S[(r + k) % K].F;
+        hit |= __builtin_amdgcn_ballot_w64(c[k] == c[k]) &
+               (__builtin_amdgcn_ballot_w64(__double2hiint(c[k]) >= F) |
+                __builtin_amdgcn_ballot_w64(F == kNegEnc));
+      } else {
+        hit |= __builtin_amdgcn_ballot_w64(col_passes(c[k], S[(r + k) % K]));
+      }
       hit |= __builtin_amdgcn_ballot_w64(c[k] >= trow);
     }
     if (hit) {
