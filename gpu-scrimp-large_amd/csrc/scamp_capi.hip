@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -398,7 +399,11 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
   // enough of them; the chunk length itself depends on one series only)
   const int64_t wave_slots = (int64_t)ctx->num_cus * 12;
   int64_t R = std::numeric_limits<int64_t>::max();
-  if (nbands * nbatch < 2 * wave_slots) {
+  // SCAMP_HIP_NO_CHUNKS (tests): never cut bands into chunks, so that short
+  // passes run the kernels of long ones
+  const char *no_chunks = std::getenv("SCAMP_HIP_NO_CHUNKS");
+  const bool may_chunk = no_chunks == nullptr || *no_chunks == '\0' || *no_chunks == '0';
+  if (may_chunk && nbands * nbatch < 2 * wave_slots) {
     const int64_t floor_rows = std::max<int64_t>(512, 4 * m);
     R = std::max(floor_rows, (total_rows + 4 * wave_slots - 1) / (4 * wave_slots));
     R = (R + 15) / 16 * 16;
