@@ -1070,15 +1070,23 @@ struct Sweep {
 
   template <bool MASKED, bool FAST>
   __device__ __forceinline__ void row(int r, int rr, int64_t i,
-                                      const Terms<DT> *s_row) {
+                                      const Terms<DT> *s_row, DT &s1_next) {
     const Terms<DT> rv = s_row[rr];
     const DT rdf = rv.df, rdg = rv.dg, rin = rv.in, trow = rv.t;
-    if constexpr (kTwoStage)
+    if constexpr (kIntCol) {
+      // the FP64 first stage's scale is read a row ahead, so that the row's
+      // first FMAs need not wait for LDS (measured 0.7 - 0.9 % faster at
+      // n = 2^22, m = 1024; the FP32 form was 1.5 % slower with it)
+      const DT s1 = s1_next;
+      if (rr + 1 < kRows) s1_next = s_row[rr + 1].s1;
+      row_two_stage<MASKED>(r, rr, i, rin, s1, trow);
+    } else if constexpr (kTwoStage) {
       row_two_stage<MASKED>(r, rr, i, rin, rv.s1, trow);
-    else if constexpr (kSumStage)
+    } else if constexpr (kSumStage) {
       row_tests<MASKED, FAST>(r, rr, i, rin, trow, rv.s1);
-    else
+    } else {
       row_tests<MASKED, FAST>(r, rr, i, rin, trow, rin);
+    }
     // advance the recurrence to row i+1
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1192,6 +1200,7 @@ struct Sweep {
       blk0 = blk1 = nan_of<DT>();
     }
     bool ended = false;
+    DT s1_next = kIntCol ? s_row[0].s1 : (DT)0;  // the first row's (row() reads ahead)
 #pragma clang loop unroll(full)
     for (int rr = 0; rr < kRows; ++rr) {
       const int r = rr % K;
@@ -1201,7 +1210,7 @@ struct Sweep {
         break;
       }
       if (KIND == kMatrix && !FAST) cells_row_start(first && r == 0);
-      row<MASKED, FAST>(r, rr, i, s_row);
+      row<MASKED, FAST>(r, rr, i, s_row, s1_next);
       if (MASKED && i + 1 == re) {
         segment_end(r, i);
         ended = true;
