@@ -933,7 +933,8 @@ struct Sweep {
     const int32_t m0 = max(max(h[0], h[1]), h[2]);
     const int32_t m1 = max(max(h[3], h[4]), h[5]);
     const int32_t mx = max(max(h[6], h[7]), max(m0, m1));
-    if (__builtin_amdgcn_ballot_w64(mx >= 0) == 0) return;
+    // (most rows stop here: the exact stage is laid out off the row's path)
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(mx >= 0) == 0, 1)) return;
     // exact stage: the correlations (cov * in_row) * in_col, NaN where the
     // diagonal is outside the pass or the column past the series.  FP64: the
     // columns' inverse norms come from memory (rows get here rarely; the
@@ -982,7 +983,7 @@ struct Sweep {
       const int32_t m0 = max(max(h[0], h[1]), h[2]);
       const int32_t m1 = max(max(h[3], h[4]), h[5]);
       const int32_t mx = max(max(h[6], h[7]), max(m0, m1));
-      if (__builtin_amdgcn_ballot_w64(mx >= 0) == 0) return;
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(mx >= 0) == 0, 1)) return;
       // the exact sums, over the correlations (cov * in_col) * in_row
       double rs = 0.0;
 #pragma unroll
