@@ -1033,7 +1033,7 @@ struct Sweep {
       }
       if (KIND == kKnn && hit) {
         knn_update(r, c, i, trow);
-      } else if (hit) {
+      } else if (__builtin_expect(hit != 0, 0)) {  // merges off the row's path
         if (ROWS) nn_row_update(c, i, rr, trow);
         if (COLS) nn_col_update(r, rr, c, i);
       }

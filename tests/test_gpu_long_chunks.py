@@ -62,6 +62,16 @@ def test_long_chunk_abjoin_both_profiles_match_oracle(no_chunks, na, nb, m):
     assert check_1nn_index(b, a, m, *split(pb), ocb, oib) == CLEAN
 
 
+@pytest.mark.parametrize("na,nb,m", [(6000, 9000, 64), (9000, 2500, 50)])
+def test_long_chunk_one_sided_abjoin_matches_oracle(no_chunks, na, nb, m):
+    # A's profile only: the one-sided long-chunk kernel (no two-stage filter)
+    a = random_walk(na, na + 7, 1)
+    b = random_walk(nb, nb + 8, 1)
+    pa, _ = C.join(a, b, C.make_params(m, has_b=True))
+    oc, oi = O.join(a, b, m)
+    assert check_1nn_index(a, b, m, *split(pa), oc, oi) == CLEAN
+
+
 @pytest.mark.parametrize("grow,gcol", [(131072, 131072), (0, 131072)])
 def test_long_chunk_aligned_abjoin_matches_oracle(no_chunks, grow, gcol):
     a = random_walk(6000, 61)
