@@ -1174,7 +1174,7 @@ struct Sweep {
   __device__ __forceinline__ bool block(int64_t i_b, int64_t re, Pre &pre,
                                         Terms<DT> *s_row, Terms<DT> *s_col,
                                         bool first) {
-    if (KIND == kMatrix && !MASKED && cells_block_fast())
+    if (KIND == kMatrix && !MASKED && __builtin_expect(cells_block_fast(), 1))  // most blocks
       return block_rows<false, true>(i_b, re, pre, s_row, s_col, first);
     return block_rows<MASKED, false>(i_b, re, pre, s_row, s_col, first);
   }
