@@ -265,11 +265,16 @@ struct Sweep {
   // (below) take longer blocks still, which spreads the block's staging and
   // row-bound reduction over more rows at the price of a looser row bound
   // and more code.  Measured in one process at n = 2^22, m = 1024 (16 / 32 /
-  // 64 rows): FP64 1719 / 1663 / 1667 ms, FP32 1423 / 1386 / 1329 ms.  A
-  // block is at most one row per lane of the staging arrays.
+  // 64 rows): FP32 1423 / 1386 / 1329 ms; FP64 (the current kernel, 32 / 64
+  // rows) 1289 / 1239 ms, where 64 rows also leave the compiler enough
+  // registers to keep the kernel free of spills.  The FP64 column-only
+  // kernel (one side of an AB-join) also runs 64-row blocks: 139 registers
+  // and no spills against 168 and 12 spilled dwords at 16 rows, AB-join
+  // 2^20 x 2^20, m = 256: 197.9 -> 194.8 ms; its FP32 twin and the row-only
+  // kernels spill more with longer blocks.  A block is at most one row per
+  // lane of the staging arrays.
   static constexpr int kRows =
-      KIND == kNN ? (!FRESH && COLS && ROWS ? (sizeof(DT) == 4 ? 8 * K : 4 * K) : 2 * K)
-                  : K;
+      KIND == kNN ? (!FRESH && COLS && (ROWS || sizeof(DT) == 8) ? 8 * K : 2 * K) : K;
   static_assert(kRows <= kWave, "one staged row and column per lane");
   // Two-stage 1NN filter (rows and columns, long chunks): every pair is
   // first tested once against min(T_lo, column threshold), T_lo the lowest
