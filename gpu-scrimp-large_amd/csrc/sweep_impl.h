@@ -271,10 +271,20 @@ struct Sweep {
   // kernel (one side of an AB-join) also runs 64-row blocks: 139 registers
   // and no spills against 168 and 12 spilled dwords at 16 rows, AB-join
   // 2^20 x 2^20, m = 256: 197.9 -> 194.8 ms; its FP32 twin and the row-only
-  // kernels spill more with longer blocks.  A block is at most one row per
-  // lane of the staging arrays.
+  // kernels spill more with longer blocks.  SUM_THRESH runs 64-row blocks
+  // (row sums still flushed every 8 rows): self-join n = 2^21, m = 256,
+  // 8 / 32 / 64 rows: FP64 467 / 426 / 406 ms, mixed 426 / 392 / 385 ms;
+  // A's sums of an AB-join of two 2^20 walks: FP64 218 -> 181 ms, mixed
+  // 179 -> 157 ms (8 -> 64 rows).  MATRIX_SUMMARY runs 32-row blocks
+  // (self-join n = 2^21, m = 1024, 1024^2 cells, 8 / 32 / 64 rows: FP64
+  // 540 / 482 / 488 ms, mixed 416 / 383 / 390 ms; a longer block is less
+  // often clear of cell boundaries).  A block is at most one row per lane
+  // of the staging arrays.
   static constexpr int kRows =
-      KIND == kNN ? (!FRESH && COLS && (ROWS || sizeof(DT) == 8) ? 8 * K : 2 * K) : K;
+      KIND == kNN ? (!FRESH && COLS && (ROWS || sizeof(DT) == 8) ? 8 * K : 2 * K)
+      : KIND == kSum ? 8 * K
+      : KIND == kMatrix ? 4 * K
+                        : K;
   static_assert(kRows <= kWave, "one staged row and column per lane");
   // Two-stage 1NN filter (rows and columns, long chunks): every pair is
   // first tested once against min(T_lo, column threshold), T_lo the lowest
@@ -669,8 +679,8 @@ struct Sweep {
   }
 
   // Fast blocks: when no cell boundary falls inside any lane's part of the
-  // next K rows (columns first .. first + 2K - 2 of the lane, rows i_b ..
-  // i_b + K - 1), every pair of a lane goes to the lane's current cell, so
+  // block's kRows rows (columns first .. first + kRows + K - 2 of the lane,
+  // rows i_b .. i_b + kRows - 1), every pair of a lane goes to the lane's current cell, so
   // the block keeps one running max per lane and folds it in at the end.
   // Wave-uniform decision; blocks that touch a boundary take the per-pair
   // path above.
@@ -678,9 +688,9 @@ struct Sweep {
     if (COLS && ROWS) return false;
     bool ok;
     if (COLS)
-      ok = dir.pos + 2 * K - 2 < a.cpc && dir_row_pos + K <= a.rpc;
+      ok = dir.pos + kRows + K - 2 < a.cpc && dir_row_pos + kRows <= a.rpc;
     else
-      ok = tra.pos + 2 * K - 2 < a.rpc && tra_col_pos + K <= a.cpc;
+      ok = tra.pos + kRows + K - 2 < a.rpc && tra_col_pos + kRows <= a.cpc;
     return __builtin_amdgcn_ballot_w64(!ok) == 0;
   }
   // A fast block needs only the largest correlation of the lane's row:
@@ -697,13 +707,13 @@ struct Sweep {
     const double v = (double)fmax(blk0, blk1);
     if (COLS) {
       dir.lo = fmax(dir.lo, v);
-      dir.pos += K;
-      dir_row_pos += K;
+      dir.pos += kRows;
+      dir_row_pos += kRows;
       if (dir_row_pos == a.rpc) dir_row_pos = 0;
     } else {
       tra.lo = fmax(tra.lo, v);
-      tra.pos += K;
-      tra_col_pos += K;
+      tra.pos += kRows;
+      tra_col_pos += kRows;
       if (tra_col_pos == a.cpc) tra_col_pos = 0;
     }
   }
@@ -1206,6 +1216,7 @@ struct Sweep {
       blk0 = blk1 = nan_of<DT>();
     }
     bool ended = false;
+    int64_t fl = i_b;  // SUM_THRESH rows: the first row whose sum is not flushed
     DT s1_next = kIntCol ? s_row[0].s1 : (DT)0;  // the first row's (row() reads ahead)
 #pragma clang loop unroll(full)
     for (int rr = 0; rr < kRows; ++rr) {
@@ -1215,7 +1226,7 @@ struct Sweep {
         ended = true;
         break;
       }
-      if (KIND == kMatrix && !FAST) cells_row_start(first && r == 0);
+      if (KIND == kMatrix && !FAST) cells_row_start(first && rr == 0);
       row<MASKED, FAST>(r, rr, i, s_row, s1_next);
       if (MASKED && i + 1 == re) {
         segment_end(r, i);
@@ -1224,8 +1235,16 @@ struct Sweep {
       }
       shift(r, rr, i, s_col);
       if (kComp && rr % kRenormRows == kRenormRows - 1) renormalise();
+      // SUM_THRESH rows: rsum holds one slot cycle (K rows); flush it at
+      // each cycle's end within the block (the last cycle's below)
+      if (KIND == kSum && ROWS && r == K - 1 && rr + 1 < kRows) {
+        flush_row_sums(fl, K);
+#pragma unroll
+        for (int q = 0; q < K; ++q) rsum[q] = 0.0;
+        fl += K;
+      }
     }
-    if (KIND == kSum && ROWS) flush_row_sums(i_b, re - i_b);
+    if (KIND == kSum && ROWS) flush_row_sums(fl, re - fl);
     if (kTable && ROWS) flush_row_table(i_b);
     if (kColTable && ct_dirty) flush_col_table(i_b);
     if (KIND == kMatrix && FAST) cells_fast_end();
