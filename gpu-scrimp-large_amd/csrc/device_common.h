@@ -92,4 +92,21 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// The FP64 two-stage 1NN filter's first-stage factor of a column with
+// inverse norm in_col: one Newton step from the hardware reciprocal
+// r = (1 + e) / in_col with 2^-30 taken off inside the step:
+// r (2 - in_col r) - 2^-30 r = (1 - e^2 - 2^-30 (1 + e)) / in_col up to two
+// roundings of 2^-53, below (1 - 2^-31) / in_col whatever the reciprocal's
+// error e (|e| < 1/4), then rounded down to FP32 (FLT_MAX at most).  +inf for
+// a column that can never match (in_col NaN).  Computed once per column with
+// the statistics (stage1_factors_kernel) and read by the sweep.
+__device__ __forceinline__ float stage1_factor(double in_col) {
+  const double r = __builtin_amdgcn_rcp(in_col);
+  const double q = fma(r, fma(-in_col, r, 1.0 - 0x1p-30), r);
+  if (!(in_col == in_col)) return __builtin_inff();
+  float f = (float)q;  // nearest; one step down where that rounded up
+  if ((double)f > q) f = __int_as_float(__float_as_int(f) - 1);
+  return f;
+}
+
 }  // namespace scamp_amd

@@ -17,6 +17,10 @@ hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
                         double *df, double *dg, int64_t stat_len,
                         hipStream_t s, int64_t batch = 1, int64_t t_stride = 0);
 
+// rq[i] = stage1_factor(inorm[i]) for i < len (the FP64 two-stage 1NN
+// filter's per-column factor)
+hipError_t launch_stage1_factors(const double *inorm, float *rq, int64_t len,
+                                 hipStream_t s);
 // FP64 -> FP32 copies of the per-window update terms and inverse norms.
 hipError_t launch_to_f32(const double *src, float *dst, int64_t len,
                          hipStream_t s);
@@ -143,6 +147,9 @@ struct SweepArgs {
   // padded by kColumnPad entries)
   const void *dfX, *dgX, *inX;
   const void *dfY, *dgY, *inY;
+  // column side X: each column's FP64 first-stage factor (stage1_factor of
+  // its FP64 inverse norm; padded like inX), read by the FP64 two-stage sweep
+  const float *rqX;
   // clean series and means (the in-kernel FP64 seeds and re-anchors)
   const double *TX, *TY, *muX, *muY;
   // work items, taken in order through `counter` and decoded in the kernel

@@ -154,6 +154,12 @@ __global__ void dfdg_kernel(const double *__restrict__ clean, int64_t m,
   }
 }
 
+__global__ void stage1_factors_kernel(const double *__restrict__ inorm,
+                                     float *__restrict__ rq, int64_t len) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) rq[i] = stage1_factor(inorm[i]);
+}
+
 __global__ void to_f32_kernel(const double *__restrict__ src,
                               float *__restrict__ dst, int64_t len) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -474,6 +480,13 @@ hipError_t launch_stats(const double *T, int64_t n, int64_t m, double *clean,
                      clean_len);
   hipLaunchKernelGGL(dfdg_kernel, dim3(blocks_for(stat_len, 256), nb), dim3(256),
                      0, s, clean, m, means, ns, df, dg, stat_len, clean_len);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage1_factors(const double *inorm, float *rq, int64_t len,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(stage1_factors_kernel, dim3(blocks_for(len, 256)), dim3(256),
+                     0, s, inorm, rq, len);
   return hipGetLastError();
 }
 

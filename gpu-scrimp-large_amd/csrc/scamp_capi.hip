@@ -229,6 +229,9 @@ struct SeriesStats {
   double *clean = nullptr, *mean = nullptr, *inorm = nullptr, *df = nullptr,
          *dg = nullptr;
   float *f_inorm = nullptr, *f_df = nullptr, *f_dg = nullptr;
+  // the FP64 sweep's per-column first-stage factors (both precisions: a
+  // mixed join whose audit fails runs again in FP64)
+  float *rq = nullptr;
 };
 
 template <class Arena>
@@ -242,6 +245,7 @@ SeriesStats alloc_stats(Arena &ar, int64_t n, int64_t m, bool fp32) {
   st.inorm = ar.template alloc<double>(stat_len);
   st.df = ar.template alloc<double>(stat_len);
   st.dg = ar.template alloc<double>(stat_len);
+  st.rq = ar.template alloc<float>(stat_len);
   if (fp32) {
     st.f_inorm = ar.template alloc<float>(stat_len);
     st.f_df = ar.template alloc<float>(stat_len);
@@ -255,6 +259,7 @@ void compute_stats(const SeriesStats &st, const double *d_T, int64_t m,
   const int64_t stat_len = st.ns + kColumnPad;
   HIP_CHECK(launch_stats(d_T, st.n, m, st.clean, st.n + kColumnPad, st.mean,
                          st.inorm, st.df, st.dg, stat_len, s));
+  HIP_CHECK(launch_stage1_factors(st.inorm, st.rq, stat_len, s));
   if (st.f_inorm) {
     HIP_CHECK(launch_to_f32(st.inorm, st.f_inorm, stat_len, s));
     HIP_CHECK(launch_to_f32(st.df, st.f_df, stat_len, s));
@@ -450,6 +455,7 @@ void run_pass(scamp_hip_ctx *ctx, uint64_t *counter, const Pass &p, int64_t m,
     a.dfX = p.X->df; a.dgX = p.X->dg; a.inX = p.X->inorm;
     a.dfY = p.Y->df; a.dgY = p.Y->dg; a.inY = p.Y->inorm;
   }
+  a.rqX = p.X->rq;
   a.TX = p.X->clean; a.TY = p.Y->clean; a.muX = p.X->mean; a.muY = p.Y->mean;
   a.nitems = nitems; a.nchunks = nchunks; a.chunk_rows = chunked ? R : 0;
   a.chunk_min_rows = rmin; a.chunk_ref_rows = ref_rows;
@@ -863,6 +869,8 @@ int selfjoin_batch(scamp_hip_ctx *ctx, const double *d_T, int64_t n,
   A.dg = ar.alloc<double>(batch * seg);
   HIP_CHECK(launch_stats(d_T, n, m, A.clean, seg, A.mean, A.inorm, A.df, A.dg,
                          seg, s, batch, n));
+  A.rq = ar.alloc<float>(batch * seg);
+  HIP_CHECK(launch_stage1_factors(A.inorm, A.rq, batch * seg, s));
   if (fp32) {
     A.f_inorm = ar.alloc<float>(batch * seg);
     A.f_df = ar.alloc<float>(batch * seg);

@@ -69,10 +69,6 @@ struct Slot {
   // instead of t.
   DT mt;
   int32_t F;
-  // FP64 two-stage filter: the column's first-stage factor (stage1_factor),
-  // moved with the column in place of in, which only the exact stage reads
-  // (from the inverse-norm array)
-  float rq;
   double acc;      // SUM: partial sum of the column, travelling with it
 };
 
@@ -90,7 +86,6 @@ struct Terms<double> {
   double df, dg, in;
   double t;
   int32_t F;  // two-stage filter: enc_thr of the column's threshold
-  float q;    // two-stage filter, columns: the first-stage factor
   double s1;  // two-stage filter, rows: the first stage's scale (in, or 0
               // for a row that can never match)
 };
@@ -126,27 +121,15 @@ __device__ __forceinline__ double mt_of(int32_t tlo, int32_t f) {
 }
 // The FP64 first stage in row-scaled units: a pair passes when the exact
 // product s = cov * in_row is at least B = fl(mt_of(tlo, f) * q - 2^-1000),
-// q = stage1_factor(in_col) (a column constant that travels with the column,
-// so that a block's refresh of B is one FMA).  q is one Newton step from the
-// hardware reciprocal r = (1 + e) / in_col with 2^-30 taken off inside the
-// step: r (2 - in_col r) - 2^-30 r = (1 - e^2 - 2^-30 (1 + e)) / in_col up to
-// two roundings of 2^-53, below (1 - 2^-31) / in_col whatever the
-// reciprocal's error e (|e| < 1/4), then rounded down to FP32 (FLT_MAX at
-// most; one DPP move per row instead of two).  So for a threshold image
-// m >= 0, B < m / in_col by a relative 2^-31 and an absolute 2^-1000, while
+// q = stage1_factor(in_col) (a column constant, kept per column in memory
+// beside the inverse norms, so that a block's refresh of B is one FMA and a
+// load issued a block ahead).  q < (1 - 2^-31) / in_col (stage1_factor).
+// So for a threshold image m >= 0, B < m / in_col by a relative 2^-31 and an absolute 2^-1000, while
 // s * in_col >= threshold >= m gives s >= m / in_col less a relative 2^-52:
 // the stage is conservative.  The negative image (-1.0625) gives B below
 // -1.0625 (1 - 2^-30) / in_col, under every correlation >= -1.  A column
 // that can never match (in_col NaN) has q = +inf and B = +inf, -inf or NaN;
 // its pairs may pass the first stage but never the exact one.
-__device__ __forceinline__ float stage1_factor(double in_col) {
-  const double r = __builtin_amdgcn_rcp(in_col);
-  const double q = fma(r, fma(-in_col, r, 1.0 - 0x1p-30), r);
-  if (!(in_col == in_col)) return __builtin_inff();
-  float f = (float)q;  // nearest; one step down where that rounded up
-  if ((double)f > q) f = __int_as_float(__float_as_int(f) - 1);
-  return f;
-}
 __device__ __forceinline__ double stage1_bound(int32_t tlo, int32_t f, float q) {
   return fma(mt_of(tlo, f), (double)q, -0x1p-1000);
 }
@@ -296,9 +279,11 @@ struct Sweep {
   // share of a max tree (below); the exact correlation s * in_col is formed
   // only on rows that pass.  With converged thresholds a few percent of rows
   // do.  The FP64 form keeps each column's threshold as an integer image
-  // (one register, one DPP move) for its exact column test, and carries the
-  // column's first-stage factor (stage1_factor) in place of in_col, which
-  // its exact stage reads from memory.
+  // (one register, one DPP move) for its exact column test; its slots carry
+  // neither in_col (the exact stage reads it from memory) nor the column's
+  // first-stage factor (stage1_factor, read from memory for each block's
+  // refresh, a block ahead): per row the hand-off moves df, dg, the bound
+  // and the image, 7 DPP moves.
   static constexpr bool kTwoStage = KIND == kNN && !FRESH && COLS && ROWS;
   static constexpr bool kIntCol = kTwoStage && sizeof(DT) == 8;
   // The first stage is tested by sign: d_k = fma(cov_k, in_row, -B_k) for
@@ -365,7 +350,6 @@ struct Sweep {
     s.df = ld<DT>(a.dfX, off);
     s.dg = ld<DT>(a.dgX, off);
     s.in = ld<DT>(a.inX, off);
-    if constexpr (kIntCol) s.rq = stage1_factor(s.in);
     if (kThr && COLS) {
       const DT t = Thresh<DT>::from_key_hi(
           ld<uint32_t>(a.keyX, static_cast<uint32_t>(col) * 8u + 4u));
@@ -384,9 +368,10 @@ struct Sweep {
 
   // Two-stage filter: the first-stage threshold of a slot from the block's
   // row bound and the column's threshold.
-  __device__ __forceinline__ void refresh_mt(Slot<DT> &s) const {
+  // q: the slot column's first-stage factor (FP64 form).
+  __device__ __forceinline__ void refresh_mt(Slot<DT> &s, float q) const {
     if constexpr (kIntCol)
-      s.mt = stage1_bound(tlo_i, s.F, s.rq);
+      s.mt = stage1_bound(tlo_i, s.F, q);
     else
       s.mt = stage1_bound32(tlo_f, s.t, s.in);
   }
@@ -395,11 +380,15 @@ struct Sweep {
   // block) and column ib+D0+kBand+l (the column entering the band at lane 63
   // on row ib+l); lanes >= kRows fetch clamped / padded entries nobody reads, so
   // the code has no divergent branch.
+  // The FP64 two-stage kernel also fetches the first-stage factors of the
+  // lane's columns on the block's first row (ib + d0 + k, slot k) for the
+  // block's bound refresh.
   struct Pre {
     DT rdf, rdg, rin;
     uint32_t rkey;
     DT cdf, cdg, cin;
     uint32_t ckey;
+    float rq[kIntCol ? K : 1];
   };
   __device__ __forceinline__ void prefetch(Pre &p, int64_t ib) const {
     int64_t ri = ib + lane;
@@ -417,6 +406,12 @@ struct Sweep {
     p.cin = ld<DT>(a.inX, coff);
     if (kThr && COLS)
       p.ckey = ld<uint32_t>(a.keyX, static_cast<uint32_t>(ci) * 8u + 4u);
+    if constexpr (kIntCol) {
+      // columns ib + d0 + k < ci: inside the padded arrays wherever ci is
+      const uint32_t qoff = static_cast<uint32_t>(ib + d0) * 4u;
+#pragma unroll
+      for (int k = 0; k < K; ++k) p.rq[k] = ld<float>(a.rqX, qoff + 4u * k);
+    }
   }
   __device__ __forceinline__ void publish(const Pre &p, Terms<DT> *s_row,
                                           Terms<DT> *s_col) {
@@ -453,8 +448,7 @@ struct Sweep {
       if constexpr (kIntCol) {
         tlo_i = rows_min(mine ? enc_thr(r.t) : kNanEnc);
         // the entering column's first-stage bound travels in its record's t
-        c.q = stage1_factor(c.in);
-        c.t = stage1_bound(tlo_i, c.F, c.q);
+        c.t = stage1_bound(tlo_i, c.F, stage1_factor(c.in));
         r.s1 = r.in == r.in ? r.in : 0.0;
         s_row[lane] = r;
         s_col[lane] = c;
@@ -1126,10 +1120,7 @@ struct Sweep {
     }
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
-    if constexpr (kIntCol)
-      s.rq = shl1(s.rq, e.q);
-    else
-      s.in = shl1(s.in, e.in);
+    if constexpr (!kIntCol) s.in = shl1(s.in, e.in);
     if constexpr (kIntCol) {
       s.F = shl1(s.F, e.F);
       s.mt = shl1(s.mt, e.t);
@@ -1201,7 +1192,7 @@ struct Sweep {
     publish(pre, s_row, s_col);
     if constexpr (kTwoStage) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) refresh_mt(S[k]);
+      for (int k = 0; k < K; ++k) refresh_mt(S[k], pre.rq[kIntCol ? k : 0]);
     }
     prefetch(pre, i_b + kRows);
     // keep the look-ahead loads where they are issued: the scheduler would
@@ -1428,6 +1419,7 @@ void sweep_kernel(SweepArgs a) {
       la.dfX = static_cast<const DT *>(a.dfX) + o;
       la.dgX = static_cast<const DT *>(a.dgX) + o;
       la.inX = static_cast<const DT *>(a.inX) + o;
+      la.rqX = a.rqX + o;
       la.dfY = static_cast<const DT *>(a.dfY) + o;
       la.dgY = static_cast<const DT *>(a.dgY) + o;
       la.inY = static_cast<const DT *>(a.inY) + o;

@@ -233,7 +233,7 @@ int scamp_hip_last_precision(scamp_hip_ctx *ctx, int *precision,
  * device's scratch array instead of hipMalloc, and the call makes no
  * hipMalloc, hipFree or event creation.  A join whose buffers exceed
  * SCAMP_HIP_NOALLOC_SCRATCH_BYTES is refused with SCAMP_HIP_INVALID_ARGS
- * (about 2.7 million subsequences for an FP64 1NN_INDEX self-join, 1.9
+ * (about 2.5 million subsequences for an FP64 1NN_INDEX self-join, 1.8
  * million for MIXED).  All calls on one GPU share its array, which keeps the
  * last call's contents (each call initialises what it reads): calls on one
  * GPU take turns under a per-device lock inside the library, and any other

@@ -95,7 +95,7 @@ def test_noalloc_matches_join_device(name, pt, prec, ab, sep, m, sizes):
 def test_noalloc_refuses_a_join_beyond_its_scratch():
     import torch
     dev = torch.device("cuda:0")
-    n, m = 3_000_000, 256  # about 144 MB of FP64 buffers: above 128 MiB
+    n, m = 3_000_000, 256  # about 156 MB of FP64 buffers: above 128 MiB
     a_d = torch.zeros(n, dtype=torch.float64, device=dev)
     prof0 = C.init_profile("1NN_INDEX", n - m + 1)
     pa = _device(prof0, dev)
