@@ -153,9 +153,9 @@ __device__ __forceinline__ int32_t sign_word(float x) { return __float_as_int(x)
 // min(tlo, t) / in_col less a relative 2^-16 (far above the FP32 rounding of
 // the product and of the hardware reciprocal) and an absolute 2^-100; +inf
 // for a column that can never match.  (Carrying a lowered reciprocal with
-// the column in place of in_col measured slower here: 1193 -> 1284 ms at
-// n = 2^22, m = 1024.  Reading in_col from memory, a block ahead for the
-// refresh and in the exact stage, instead of carrying it: 1146 -> 1116 ms.)
+// the column as the FP64 form does, with the exact stage reading in_col from
+// memory, measured slower here: 1193 -> 1284 ms at n = 2^22, m = 1024; the
+// FP32 form's 64-row blocks spend half as much per row on the refresh.)
 __device__ __forceinline__ float stage1_bound32(float tlo, float t, float in_col) {
   const float b = min_thr(tlo, t) * __builtin_amdgcn_rcpf(in_col);
   const float lowered = fmaf(-fabsf(b), 0x1p-16f, b) - 0x1p-100f;
@@ -279,12 +279,11 @@ struct Sweep {
   // share of a max tree (below); the exact correlation s * in_col is formed
   // only on rows that pass.  With converged thresholds a few percent of rows
   // do.  The FP64 form keeps each column's threshold as an integer image
-  // (one register, one DPP move) for its exact column test.  The slots of
-  // both forms carry no in_col: the exact stage reads it from memory, and
-  // each block's refresh takes a per-column value loaded a block ahead (FP64:
-  // the column's first-stage factor, stage1_factor; FP32: in_col itself).
-  // Per row the hand-off moves df, dg, the bound and the threshold: 7 DPP
-  // moves in FP64, 4 in FP32.
+  // (one register, one DPP move) for its exact column test; its slots carry
+  // neither in_col (the exact stage reads it from memory) nor the column's
+  // first-stage factor (stage1_factor, read from memory for each block's
+  // refresh, a block ahead): per row the hand-off moves df, dg, the bound
+  // and the image, 7 DPP moves.
   static constexpr bool kTwoStage = KIND == kNN && !FRESH && COLS && ROWS;
   static constexpr bool kIntCol = kTwoStage && sizeof(DT) == 8;
   // The first stage is tested by sign: d_k = fma(cov_k, in_row, -B_k) for
@@ -369,27 +368,27 @@ struct Sweep {
 
   // Two-stage filter: the first-stage threshold of a slot from the block's
   // row bound and the column's threshold.
-  // q: the slot column's first-stage factor (FP64) or inverse norm (FP32).
+  // q: the slot column's first-stage factor (FP64 form).
   __device__ __forceinline__ void refresh_mt(Slot<DT> &s, float q) const {
     if constexpr (kIntCol)
       s.mt = stage1_bound(tlo_i, s.F, q);
     else
-      s.mt = stage1_bound32(tlo_f, s.t, q);
+      s.mt = stage1_bound32(tlo_f, s.t, s.in);
   }
 
   // Terms staged a block ahead: lane l fetches row ib+l (the rows of the
   // block) and column ib+D0+kBand+l (the column entering the band at lane 63
   // on row ib+l); lanes >= kRows fetch clamped / padded entries nobody reads, so
   // the code has no divergent branch.
-  // The two-stage kernels also fetch, for the lane's columns on the block's
-  // first row (ib + d0 + k, slot k), what the block's bound refresh needs:
-  // the first-stage factor (FP64) or the inverse norm (FP32).
+  // The FP64 two-stage kernel also fetches the first-stage factors of the
+  // lane's columns on the block's first row (ib + d0 + k, slot k) for the
+  // block's bound refresh.
   struct Pre {
     DT rdf, rdg, rin;
     uint32_t rkey;
     DT cdf, cdg, cin;
     uint32_t ckey;
-    float rq[kTwoStage ? K : 1];
+    float rq[kIntCol ? K : 1];
   };
   __device__ __forceinline__ void prefetch(Pre &p, int64_t ib) const {
     int64_t ri = ib + lane;
@@ -407,12 +406,11 @@ struct Sweep {
     p.cin = ld<DT>(a.inX, coff);
     if (kThr && COLS)
       p.ckey = ld<uint32_t>(a.keyX, static_cast<uint32_t>(ci) * 8u + 4u);
-    if constexpr (kTwoStage) {
+    if constexpr (kIntCol) {
       // columns ib + d0 + k < ci: inside the padded arrays wherever ci is
       const uint32_t qoff = static_cast<uint32_t>(ib + d0) * 4u;
-      const void *qsrc = kIntCol ? static_cast<const void *>(a.rqX) : a.inX;
 #pragma unroll
-      for (int k = 0; k < K; ++k) p.rq[k] = ld<float>(qsrc, qoff + 4u * k);
+      for (int k = 0; k < K; ++k) p.rq[k] = ld<float>(a.rqX, qoff + 4u * k);
     }
   }
   __device__ __forceinline__ void publish(const Pre &p, Terms<DT> *s_row,
@@ -954,7 +952,7 @@ struct Sweep {
     const uint32_t coff = static_cast<uint32_t>(i + d0) * sizeof(DT);
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      inc[k] = ld<DT>(a.inX, coff + k * (uint32_t)sizeof(DT));
+      inc[k] = kIntCol ? ld<DT>(a.inX, coff + k * (uint32_t)sizeof(DT)) : S[(r + k) % K].in;
     DT c[K];
     uint64_t hit = 0;
 #pragma unroll
@@ -1122,7 +1120,7 @@ struct Sweep {
     }
     s.df = shl1(s.df, e.df);
     s.dg = shl1(s.dg, e.dg);
-    if constexpr (!kTwoStage) s.in = shl1(s.in, e.in);
+    if constexpr (!kIntCol) s.in = shl1(s.in, e.in);
     if constexpr (kIntCol) {
       s.F = shl1(s.F, e.F);
       s.mt = shl1(s.mt, e.t);
@@ -1194,7 +1192,7 @@ struct Sweep {
     publish(pre, s_row, s_col);
     if constexpr (kTwoStage) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) refresh_mt(S[k], pre.rq[k]);
+      for (int k = 0; k < K; ++k) refresh_mt(S[k], pre.rq[kIntCol ? k : 0]);
     }
     prefetch(pre, i_b + kRows);
     // keep the look-ahead loads where they are issued: the scheduler would
